@@ -1,0 +1,765 @@
+// The C ABI's batch calls under `dinara_costs`: validation with the reference's result codes, packing into the
+// device tape, the sweep launches class by class until every pair is proven, and results in the caller's
+// order. Reference behaviour followed: c/dinara.mojo:111-190 (argument parsing and codes), :96-108 (per-pair
+// refusals), :563-819 (batch results), lanes.mojo:1163-1252 (first band), api.mojo:1188-1213 (alignments).
+
+#include <algorithm>
+#include <functional>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "../../include/dinara.h"
+#include "common.h"
+#include "device.h"
+
+using namespace dinara;
+
+namespace {
+
+constexpr int64_t UNBOUNDED = int64_t(1) << 60;
+constexpr int64_t MAX_COST = int64_t(1) << 32;
+
+// The costs a batch asked for, checked as the reference checks them (modes.mojo:84-143); `code` is nonzero
+// when the batch fails whole.
+struct ParsedCosts {
+    int64_t code = 0;
+    int64_t mismatch = 1, opening = 0, extension = 1, del_opening = 0, del_extension = 1;
+    bool two_piece = false;
+};
+
+ParsedCosts parse_costs(const dinara_costs *c) {
+    ParsedCosts p;
+    if (!c) return p;  // unit costs
+    auto invalid = [&]() {
+        p.code = DINARA_INVALID_COSTS;
+        return p;
+    };
+    if (c->mismatch <= 0 || c->extension <= 0 || c->opening < 0) return invalid();
+    if (c->mismatch > MAX_COST || c->opening > MAX_COST || c->extension > MAX_COST) return invalid();
+    p.mismatch = c->mismatch;
+    p.opening = c->opening;
+    p.extension = c->extension;
+    p.del_opening = c->opening;
+    p.del_extension = c->extension;
+    if (c->opening2 >= 0) {
+        if (c->extension2 <= 0) return invalid();
+        if (c->opening2 > MAX_COST || c->extension2 > MAX_COST) return invalid();
+        p.two_piece = true;
+    }
+    if (c->deletion_extension != 0) {
+        if (c->deletion_extension <= 0 || c->deletion_opening < 0) return invalid();
+        if (c->deletion_opening2 >= 0 && c->deletion_extension2 <= 0) return invalid();
+        if (c->deletion_opening > MAX_COST || c->deletion_extension > MAX_COST) return invalid();
+        if (c->deletion_opening2 > MAX_COST || c->deletion_extension2 > MAX_COST) return invalid();
+        p.del_opening = c->deletion_opening;
+        p.del_extension = c->deletion_extension;
+        if (c->deletion_opening2 >= 0) p.two_piece = true;
+    }
+    return p;
+}
+
+// The mode: global is all this library serves under Costs. Invalid modes are refused as the reference refuses
+// them (modes.mojo:261-386, c/dinara.mojo:130-151); valid ones it does not serve as DINARA_INVALID_MODE.
+int64_t check_mode(const dinara_mode *m) {
+    if (!m) return 0;
+    if (m->kind == DINARA_EXTENSION) {
+        if (m->match_score < 0 || m->match_score > MAX_COST) return DINARA_INVALID_COSTS;
+        return DINARA_INVALID_MODE;
+    }
+    if (m->kind == DINARA_LOCAL) {
+        if (m->match_score < 0 || m->match_score > MAX_COST) return DINARA_INVALID_COSTS;
+        return DINARA_INVALID_MODE;
+    }
+    if (m->kind != DINARA_ENDS_FREE) return DINARA_INVALID_MODE;
+    if (std::min(std::min(m->reference_start, m->reference_end), std::min(m->query_start, m->query_end)) < 0)
+        return DINARA_INVALID_MODE;
+    if (m->match_score < 0 || m->match_score > MAX_COST) return DINARA_INVALID_COSTS;
+    const bool global = (m->reference_start | m->reference_end | m->query_start | m->query_end) == 0;
+    if (!global || m->match_score > 0) return DINARA_INVALID_MODE;
+    return 0;
+}
+
+struct Options {
+    int64_t band_low = -UNBOUNDED, band_high = UNBOUNDED;  // library terms: reference less query letters
+    int64_t max_cost = -1;
+    bool eqx = true;
+    bool right_ties = false;
+};
+
+Options parse_options(const dinara_options *o) {
+    Options p;
+    if (!o) return p;
+    p.band_low = std::min(std::max(o->band_low, -UNBOUNDED), UNBOUNDED);
+    p.band_high = std::min(std::max(o->band_high, -UNBOUNDED), UNBOUNDED);
+    p.max_cost = o->max_cost;
+    p.eqx = o->eqx != 0;
+    p.right_ties = o->right_ties != 0;
+    return p;
+}
+
+// Zero for a pair the library takes, else why not (reference c/dinara.mojo:96-108).
+int64_t pair_code(const char *ref, int64_t n, const char *qry, int64_t m) {
+    if (n < 0 || m < 0) return DINARA_INVALID_LENGTH;
+    auto plain = [](const char *s, int64_t len) {
+        const unsigned char *u = reinterpret_cast<const unsigned char *>(s);
+        for (int64_t i = 0; i < len; ++i)
+            if (u[i] >= 0xFE) return false;
+        return true;
+    };
+    if ((n > 0 && !plain(ref, n)) || (m > 0 && !plain(qry, m))) return DINARA_UNSUPPORTED_SYMBOLS;
+    return 0;
+}
+
+void parallel_for(int64_t count, int64_t threads, const std::function<void(int64_t, int64_t)> &body) {
+    const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(threads, count));
+    if (workers <= 1) {
+        body(0, count);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int64_t w = 0; w < workers; ++w)
+        pool.emplace_back([&, w]() { body(count * w / workers, count * (w + 1) / workers); });
+    for (auto &t : pool) t.join();
+}
+
+int first_class_for(int need, bool lanes) {
+    if (lanes) {
+        if (need <= 16) return CLASS_LANE16;
+        if (need <= 32) return CLASS_LANE32;
+        if (need <= 64) return CLASS_LANE64;
+    } else if (need <= 64) {
+        return CLASS_WAVE_Q1;
+    }
+    for (int cls = CLASS_WAVE_FIRST; cls < CLASS_WAVE_FIRST + 5; ++cls)
+        if (class_width(cls) >= need) return cls;
+    return CLASS_TOO_WIDE;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ the batch
+
+struct dinara_batch {
+    int gpu = 0;
+    int64_t kind = DINARA_BATCH_DISTANCES;
+    int64_t pairs = 0;
+    int64_t threads = 0;
+    Options options;
+    ParsedCosts parsed;
+    CostParams costs{};
+    bool user_band = false;
+    int user_low = 0, user_high = 0;
+    int initial_width = 32;
+    std::vector<int64_t> code;            // per pair: 0 on the device, else its own result code
+    std::vector<int64_t> reference_length, query_length;
+    std::vector<int32_t> slot_pair;       // slot -> pair
+    std::vector<int32_t> rows, cols;      // per slot
+    std::vector<int32_t> first_low, first_high;
+    std::vector<int32_t> first_lists[CLASSES];
+    std::vector<int64_t> scratch_base;
+    int bits = 2;
+    size_t lds_limit = 0;
+    DeviceContext *ctx = nullptr;
+    // device memory
+    DeviceBuffer tape, group_base, group_ref_words, rows_d, cols_d, want_low, want_high, first_low_d,
+        first_high_d, cost, status, cigar_scratch, cigar_scratch_base, cigars, cigar_used, cigar_offset,
+        cigar_length, retry_lists, retry_counts, list_d, flag_base_d, flags;
+    // the last run's results, by slot
+    std::vector<int32_t> cost_h, status_h, cigar_length_h;
+    std::vector<int64_t> cigar_offset_h;
+    std::vector<char> cigars_h;
+    bool ran = false;
+    // statistics of the last run
+    double run_ms = 0, sweep_ms = 0, launches = 0, cells = 0, first_proven = 0;
+    double class_ms[CLASSES] = {}, class_launches[CLASSES] = {}, class_cells[CLASSES] = {},
+           class_pairs[CLASSES] = {};
+};
+
+namespace {
+
+// Builds the tape: pairs by length, longest first, 64 to a group, each group's words transposed.
+int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qrys) {
+    const int64_t pairs = b->pairs;
+    std::vector<int32_t> order;
+    order.reserve(pairs);
+    for (int64_t i = 0; i < pairs; ++i)
+        if (b->code[i] == 0) order.push_back((int32_t)i);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return std::max(b->reference_length[x], b->query_length[x]) >
+               std::max(b->reference_length[y], b->query_length[y]);
+    });
+    const int64_t slots = (int64_t)order.size();
+    b->slot_pair = order;
+    if (slots == 0) return 0;  // every pair answered by its own code
+    b->rows.resize(slots);
+    b->cols.resize(slots);
+    for (int64_t s = 0; s < slots; ++s) {
+        b->rows[s] = (int32_t)b->reference_length[order[s]];
+        b->cols[s] = (int32_t)b->query_length[order[s]];
+    }
+    // Two bits a letter when every pair holds at most four distinct bytes (each pair its own codes: only
+    // equality counts), else a byte a letter.
+    std::atomic<bool> wide{false};
+    parallel_for(slots, b->threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t s = lo; s < hi && !wide.load(std::memory_order_relaxed); ++s) {
+            const int32_t p = order[s];
+            bool seen[256] = {false};
+            int distinct = 0;
+            auto scan = [&](const char *text, int64_t len) {
+                const unsigned char *u = reinterpret_cast<const unsigned char *>(text);
+                for (int64_t i = 0; i < len && distinct <= 4; ++i)
+                    if (!seen[u[i]]) {
+                        seen[u[i]] = true;
+                        ++distinct;
+                    }
+            };
+            scan(refs[p], b->reference_length[p]);
+            scan(qrys[p], b->query_length[p]);
+            if (distinct > 4) wide.store(true);
+        }
+    });
+    b->bits = wide.load() ? 8 : 2;
+    const int per = 32 / b->bits;
+    const int64_t groups = (slots + 63) / 64;
+    std::vector<int64_t> gbase(std::max<int64_t>(groups, 1));
+    std::vector<int32_t> gref(std::max<int64_t>(groups, 1)), gqry(std::max<int64_t>(groups, 1));
+    int64_t words = 0;
+    for (int64_t g = 0; g < groups; ++g) {
+        int32_t rw = 0, qw = 0;
+        for (int64_t s = g * 64; s < std::min(slots, (g + 1) * 64); ++s) {
+            rw = std::max<int32_t>(rw, (b->rows[s] + per - 1) / per);
+            qw = std::max<int32_t>(qw, (b->cols[s] + per - 1) / per);
+        }
+        gbase[g] = words;
+        gref[g] = rw;
+        gqry[g] = qw;
+        words += (int64_t)(rw + qw) * 64;
+    }
+    std::vector<uint32_t> tape(std::max<int64_t>(words, 1), 0);
+    // Ties.LEFT is the rule over both sequences reversed, so their tape is reversed (lanes.mojo:1523-1525).
+    const bool reversed = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
+    parallel_for(groups, b->threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t g = lo; g < hi; ++g) {
+            for (int64_t s = g * 64; s < std::min(slots, (g + 1) * 64); ++s) {
+                const int32_t p = order[s];
+                int16_t codes[256];
+                for (int i = 0; i < 256; ++i) codes[i] = -1;
+                int next = 0;
+                auto code_of = [&](unsigned char byte) -> uint32_t {
+                    if (b->bits == 8) return byte;
+                    if (codes[byte] < 0) codes[byte] = (int16_t)next++;
+                    return (uint32_t)codes[byte];
+                };
+                uint32_t *ref_out = tape.data() + gbase[g] + (s & 63);
+                uint32_t *qry_out = ref_out + (int64_t)gref[g] * 64;
+                auto pack = [&](const char *text, int64_t len, uint32_t *out) {
+                    const unsigned char *u = reinterpret_cast<const unsigned char *>(text);
+                    for (int64_t i = 0; i < len; ++i) {
+                        const unsigned char byte = reversed ? u[len - 1 - i] : u[i];
+                        out[(i / per) * 64] |= code_of(byte) << (b->bits * (i % per));
+                    }
+                };
+                pack(refs[p], b->reference_length[p], ref_out);
+                pack(qrys[p], b->query_length[p], qry_out);
+            }
+        }
+    });
+    // The first band: the lanes' start and end diagonals and one either side (lanes.mojo:1207-1215), within the
+    // user band; swept at no less than the initial width, centred.
+    b->first_low.resize(slots);
+    b->first_high.resize(slots);
+    for (auto &l : b->first_lists) l.clear();
+    b->scratch_base.resize(slots + 1);
+    int64_t scratch = 0;
+    for (int64_t s = 0; s < slots; ++s) {
+        const int32_t n = b->rows[s], m = b->cols[s];
+        const int32_t end = m - n;
+        int32_t lo = std::min(0, end) - 1, hi = std::max(0, end) + 1;
+        bool lanes = true;
+        if (b->user_band) {
+            lo = std::max(lo, b->user_low);
+            hi = std::min(hi, b->user_high);
+            lanes = b->user_low <= -n && b->user_high >= m;  // the band leaves this pair whole
+        }
+        b->first_low[s] = lo;
+        b->first_high[s] = hi;
+        const int need = std::max(hi - lo + 1, b->initial_width);
+        const int cls = first_class_for(need, lanes);
+        // A pair whose start and end diagonals lie further apart than the widest band is left unswept: its
+        // status stays unsettled and it reports DINARA_OUT_OF_MEMORY.
+        if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
+        b->scratch_base[s] = scratch;
+        if (b->kind == DINARA_BATCH_ALIGNMENTS) scratch += 2 * ((int64_t)n + m) + 2;
+    }
+    b->scratch_base[slots] = scratch;
+
+    // Upload.
+    DeviceContext *ctx = b->ctx;
+    if (!b->tape.upload(ctx, tape.data(), tape.size() * 4) ||
+        !b->group_base.upload(ctx, gbase.data(), gbase.size() * 8) ||
+        !b->group_ref_words.upload(ctx, gref.data(), gref.size() * 4) ||
+        !b->rows_d.upload(ctx, b->rows.data(), std::max<int64_t>(slots, 1) * 4) ||
+        !b->cols_d.upload(ctx, b->cols.data(), std::max<int64_t>(slots, 1) * 4) ||
+        !b->first_low_d.upload(ctx, b->first_low.data(), std::max<int64_t>(slots, 1) * 4) ||
+        !b->first_high_d.upload(ctx, b->first_high.data(), std::max<int64_t>(slots, 1) * 4) ||
+        !b->want_low.reserve(std::max<int64_t>(slots, 1) * 4) || !b->want_high.reserve(std::max<int64_t>(slots, 1) * 4) ||
+        !b->cost.reserve(std::max<int64_t>(slots, 1) * 4) || !b->status.reserve(std::max<int64_t>(slots, 1) * 4) ||
+        !b->retry_lists.reserve((size_t)CLASSES * std::max<int64_t>(slots, 1) * 4) ||
+        !b->retry_counts.reserve(CLASSES * 4) || !b->list_d.reserve(std::max<int64_t>(slots, 1) * 4 + 64) ||
+        !b->flag_base_d.reserve(std::max<int64_t>(slots, 1) * 8 + 64))
+        return DINARA_OUT_OF_MEMORY;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS) {
+        if (!b->cigar_scratch.reserve(std::max<int64_t>(scratch, 1)) ||
+            !b->cigar_scratch_base.upload(ctx, b->scratch_base.data(), b->scratch_base.size() * 8) ||
+            !b->cigars.reserve(std::max<int64_t>(scratch, 1)) || !b->cigar_used.reserve(8) ||
+            !b->cigar_offset.reserve(std::max<int64_t>(slots, 1) * 8) ||
+            !b->cigar_length.reserve(std::max<int64_t>(slots, 1) * 4))
+            return DINARA_OUT_OF_MEMORY;
+    }
+    return 0;
+}
+
+SweepArgs sweep_args(dinara_batch *b) {
+    SweepArgs a{};
+    a.tape.words = static_cast<const uint32_t *>(b->tape.ptr);
+    a.tape.group_base = static_cast<const int64_t *>(b->group_base.ptr);
+    a.tape.group_ref_words = static_cast<const int32_t *>(b->group_ref_words.ptr);
+    a.tape.rows = static_cast<const int32_t *>(b->rows_d.ptr);
+    a.tape.cols = static_cast<const int32_t *>(b->cols_d.ptr);
+    a.tape.bits = b->bits;
+    a.want_low = static_cast<int32_t *>(b->want_low.ptr);
+    a.want_high = static_cast<int32_t *>(b->want_high.ptr);
+    a.costs = b->costs;
+    a.strict = b->kind == DINARA_BATCH_ALIGNMENTS;
+    a.cap = b->options.max_cost >= 0 ? b->options.max_cost : INT64_MAX / 4;
+    a.user_band = b->user_band;
+    a.user_low = b->user_low;
+    a.user_high = b->user_high;
+    a.eqx = b->options.eqx;
+    a.forward_cigar = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
+    a.record = b->kind == DINARA_BATCH_ALIGNMENTS;
+    a.cost = static_cast<int32_t *>(b->cost.ptr);
+    a.status = static_cast<int32_t *>(b->status.ptr);
+    a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
+    a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
+    a.cigars = static_cast<char *>(b->cigars.ptr);
+    a.cigar_used = static_cast<unsigned long long *>(b->cigar_used.ptr);
+    a.cigar_offset = static_cast<int64_t *>(b->cigar_offset.ptr);
+    a.cigar_length = static_cast<int32_t *>(b->cigar_length.ptr);
+    a.retry_lists = static_cast<int32_t *>(b->retry_lists.ptr);
+    a.retry_counts = static_cast<int32_t *>(b->retry_counts.ptr);
+    a.retry_capacity = (int)std::max<size_t>(b->slot_pair.size(), 1);
+    return a;
+}
+
+constexpr size_t FLAG_BUDGET = size_t(8) << 30;  // decisions one launch may keep
+
+// Sweeps one class over `list`, in launches whose decisions fit the budget.
+int64_t sweep_class(dinara_batch *b, int cls, const std::vector<int32_t> &list) {
+    DeviceContext *ctx = b->ctx;
+    SweepArgs a = sweep_args(b);
+    const bool wave = class_is_wave(cls);
+    const int width = class_width(cls);
+    const bool record = a.record;
+    size_t start = 0;
+    while (start < list.size()) {
+        // Take blocks (64 pairs, lanes) or pairs (waves) while their decisions fit.
+        std::vector<int64_t> bases;
+        size_t stop = start;
+        size_t bytes = 0;
+        size_t lds = 0;
+        const size_t step = wave ? 1 : 64;
+        while (stop < list.size()) {
+            const size_t next = std::min(list.size(), stop + step);
+            size_t need = 0;
+            if (record) {
+                if (wave) {
+                    need = ((size_t)b->rows[list[stop]] * width + 3) & ~size_t(3);
+                } else {
+                    int32_t rows = 0;
+                    for (size_t i = stop; i < next; ++i) rows = std::max(rows, b->rows[list[i]]);
+                    need = (size_t)rows * (width / 8) * 64 * 4;
+                }
+            }
+            if (stop > start && bytes + need > FLAG_BUDGET) break;
+            bases.push_back(wave ? (int64_t)bytes : (int64_t)(bytes / 4));
+            bytes += need;
+            if (wave) lds = std::max(lds, (size_t)b->rows[list[stop]] + (size_t)b->cols[list[stop]]);
+            stop = next;
+        }
+        const int count = (int)(stop - start);
+        if (wave && lds > b->lds_limit) return DINARA_OUT_OF_MEMORY;
+        if (!b->list_d.write(ctx, list.data() + start, count * 4) ||
+            !b->flag_base_d.write(ctx, bases.data(), bases.size() * 8))
+            return DINARA_OUT_OF_MEMORY;
+        if (record && !b->flags.reserve(std::max<size_t>(bytes, 4))) return DINARA_OUT_OF_MEMORY;
+        a.list = static_cast<const int32_t *>(b->list_d.ptr);
+        a.count = count;
+        a.flags = static_cast<uint32_t *>(b->flags.ptr);
+        a.flag_base = static_cast<const int64_t *>(b->flag_base_d.ptr);
+        launch_sweep(cls, a, wave ? std::max<size_t>(lds, 4) : 0, ctx->stream);
+        if (hipGetLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        b->launches += 1;
+        b->class_launches[cls] += 1;
+        b->class_pairs[cls] += count;
+        for (size_t i = start; i < stop; ++i) {
+            b->cells += (double)b->rows[list[i]] * width;
+            b->class_cells[cls] += (double)b->rows[list[i]] * width;
+        }
+        start = stop;
+    }
+    return 0;
+}
+
+int64_t run(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    DeviceGuard guard(ctx);
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    b->launches = b->cells = b->first_proven = 0;
+    b->run_ms = b->sweep_ms = 0;
+    for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_cells[c] = b->class_pairs[c] = 0;
+    hipEventRecord(ctx->start, ctx->stream);
+    if (slots > 0) {
+        hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream);
+        hipMemcpyAsync(b->want_high.ptr, b->first_high_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream);
+        if (b->kind == DINARA_BATCH_ALIGNMENTS) hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream);
+        hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream);
+    }
+    std::vector<int32_t> lists[CLASSES];
+    for (int c = 0; c < CLASSES; ++c) lists[c] = b->first_lists[c];
+    bool first = true;
+    for (int round = 0; slots > 0; ++round) {
+        bool any = false;
+        for (int c = 0; c < CLASSES; ++c) any = any || !lists[c].empty();
+        if (!any || round > 64) break;
+        hipMemsetAsync(b->retry_counts.ptr, 0, CLASSES * 4, ctx->stream);
+        hipEventRecord(ctx->sweep_start, ctx->stream);
+        // Each class's launches between two events of their own, for the per-kernel time.
+        for (int c = 0; c < CLASSES; ++c) {
+            if (lists[c].empty()) continue;
+            hipEventRecord(ctx->event(2 * c), ctx->stream);
+            const int64_t code = sweep_class(b, c, lists[c]);
+            if (code) return code;
+            hipEventRecord(ctx->event(2 * c + 1), ctx->stream);
+        }
+        hipEventRecord(ctx->sweep_stop, ctx->stream);
+        int32_t counts[CLASSES];
+        if (hipMemcpyAsync(counts, b->retry_counts.ptr, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return DINARA_OUT_OF_MEMORY;
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->sweep_start, ctx->sweep_stop);
+        b->sweep_ms += ms;
+        for (int c = 0; c < CLASSES; ++c) {
+            if (lists[c].empty()) continue;
+            float class_ms = 0;
+            hipEventElapsedTime(&class_ms, ctx->event(2 * c), ctx->event(2 * c + 1));
+            b->class_ms[c] += class_ms;
+        }
+        int64_t retried = 0;
+        for (int c = 0; c < CLASSES; ++c) {
+            lists[c].resize(counts[c]);
+            retried += counts[c];
+            if (counts[c] > 0)
+                hipMemcpyAsync(lists[c].data(), static_cast<int32_t *>(b->retry_lists.ptr) + (size_t)c * std::max<int64_t>(slots, 1),
+                               counts[c] * 4, hipMemcpyDeviceToHost, ctx->stream);
+        }
+        if (first) b->first_proven = (double)(slots - retried);
+        first = false;
+        if (retried > 0 && hipStreamSynchronize(ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        // A lane launch's list holds a block's worth of slots in length order; keep retries so too.
+        for (int c = 0; c < CLASSES; ++c) std::sort(lists[c].begin(), lists[c].end());
+    }
+    hipEventRecord(ctx->stop, ctx->stream);
+    if (hipEventSynchronize(ctx->stop) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    float total = 0;
+    hipEventElapsedTime(&total, ctx->start, ctx->stop);
+    b->run_ms = total;
+    b->ran = true;
+    return 0;
+}
+
+int64_t fetch(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    DeviceGuard guard(ctx);
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    b->cost_h.resize(slots);
+    b->status_h.resize(slots);
+    if (slots == 0) return 0;
+    if (!b->cost.read(ctx, b->cost_h.data(), slots * 4) || !b->status.read(ctx, b->status_h.data(), slots * 4))
+        return DINARA_OUT_OF_MEMORY;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS) {
+        unsigned long long used = 0;
+        b->cigar_offset_h.resize(slots);
+        b->cigar_length_h.resize(slots);
+        if (!b->cigar_used.read(ctx, &used, 8) || !b->cigar_offset.read(ctx, b->cigar_offset_h.data(), slots * 8) ||
+            !b->cigar_length.read(ctx, b->cigar_length_h.data(), slots * 4))
+            return DINARA_OUT_OF_MEMORY;
+        b->cigars_h.resize(std::max<unsigned long long>(used, 1));
+        if (used > 0 && !b->cigars.read(ctx, b->cigars_h.data(), used)) return DINARA_OUT_OF_MEMORY;
+    }
+    return 0;
+}
+
+// A pair's code when the device left it: under a cap, a band no alignment fits leaves nothing within it.
+int64_t unsettled_code(const dinara_batch *b, int32_t status) {
+    if (status == 2) return DINARA_OUT_OF_MEMORY;
+    return DINARA_OUT_OF_MEMORY;
+}
+
+dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *const *refs, const int64_t *ref_lens,
+                        const char *const *qrys, const int64_t *qry_lens, const dinara_costs *costs,
+                        const dinara_mode *mode, const dinara_options *options, int64_t threads, int64_t *status) {
+    auto fail = [&](int64_t code) -> dinara_batch * {
+        if (status) *status = code;
+        return nullptr;
+    };
+    ParsedCosts parsed = parse_costs(costs);
+    if (parsed.code) return fail(parsed.code);
+    if (const int64_t code = check_mode(mode)) return fail(code);
+    if (parsed.two_piece) return fail(DINARA_INVALID_MODE);
+    if (kind != DINARA_BATCH_DISTANCES && kind != DINARA_BATCH_ALIGNMENTS) return fail(DINARA_INVALID_MODE);
+    DeviceContext *ctx = device_context(gpu);
+    if (!ctx) return fail(DINARA_NO_DEVICE);
+    auto *b = new dinara_batch();
+    b->gpu = (int)gpu;
+    b->ctx = ctx;
+    b->kind = kind;
+    b->pairs = std::max<int64_t>(pairs, 0);
+    b->threads = threads;
+    b->parsed = parsed;
+    b->options = parse_options(options);
+    b->lds_limit = ctx->lds_limit;
+    if (const char *w = std::getenv("DINARA_INITIAL_WIDTH")) b->initial_width = std::max(1, atoi(w));
+    CostParams &c = b->costs;
+    c.mismatch = (int)parsed.mismatch;
+    c.del_open = (int)parsed.del_opening;
+    c.del_ext = (int)parsed.del_extension;
+    c.ins_open = (int)parsed.opening;
+    c.ins_ext = (int)parsed.extension;
+    c.del_first = c.del_open + c.del_ext;
+    c.ins_first = c.ins_open + c.ins_ext;
+    // The lanes' diagonals are the query's letters less the reference's: the library band mirrored
+    // (lanes.mojo:1151-1155).
+    const Options &o = b->options;
+    b->user_band = o.band_low > -UNBOUNDED || o.band_high < UNBOUNDED;
+    const int64_t clamp = int64_t(1) << 30;
+    b->user_low = (int)std::max<int64_t>(-o.band_high, -clamp);
+    b->user_high = (int)std::min<int64_t>(-o.band_low, clamp);
+    const int64_t dearest = std::max<int64_t>(
+        parsed.mismatch, std::max(parsed.opening + parsed.extension, parsed.del_opening + parsed.del_extension));
+    b->code.assign(b->pairs, 0);
+    b->reference_length.assign(b->pairs, 0);
+    b->query_length.assign(b->pairs, 0);
+    const bool capped = o.max_cost >= 0;
+    parallel_for(b->pairs, threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            const int64_t n = ref_lens[i], m = qry_lens[i];
+            int64_t code = pair_code(refs[i], n, qrys[i], m);
+            if (code == 0) {
+                // Every cost the sweep holds stays well inside 32 bits.
+                if ((n + m + 4) * dearest >= REAL_LIMIT || n >= (int64_t(1) << 28) || m >= (int64_t(1) << 28))
+                    code = DINARA_OUT_OF_MEMORY;
+            }
+            if (code == 0 && b->user_band) {
+                // No alignment inside a band missing the start's or the end's diagonal (lanes.mojo:1363-1367).
+                const int64_t end = m - n;
+                if (b->user_low > 0 || b->user_high < 0 || b->user_low > end || b->user_high < end)
+                    code = capped ? DINARA_ABOVE_MAX : DINARA_OUTSIDE_BAND;
+            }
+            b->code[i] = code;
+            b->reference_length[i] = std::max<int64_t>(n, 0);
+            b->query_length[i] = std::max<int64_t>(m, 0);
+        }
+    });
+    DeviceGuard guard(ctx);
+    const int64_t code = prepare(b, refs, qrys);
+    if (code) {
+        delete b;
+        return fail(code);
+    }
+    if (status) *status = 0;
+    return b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ the C ABI
+
+extern "C" {
+
+void dinara_free(char *cigar) { std::free(cigar); }
+
+int64_t dinara_device_count(void) { return device_count(); }
+
+dinara_batch *dinara_batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *const *references,
+                               const int64_t *reference_lengths, const char *const *queries,
+                               const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                               const dinara_options *options, int64_t threads, int64_t *status) {
+    return batch_new(gpu, kind, pairs, references, reference_lengths, queries, query_lengths, costs, mode, options,
+                     threads, status);
+}
+
+int64_t dinara_batch_run(dinara_batch *batch) {
+    if (!batch) return DINARA_INVALID_MODE;
+    return run(batch);
+}
+
+int64_t dinara_batch_distances(dinara_batch *b, int64_t *results) {
+    if (!b || b->kind != DINARA_BATCH_DISTANCES || !b->ran) return DINARA_INVALID_MODE;
+    if (const int64_t code = fetch(b)) return code;
+    const bool capped = b->options.max_cost >= 0;
+    for (int64_t i = 0; i < b->pairs; ++i) results[i] = b->code[i];
+    for (size_t s = 0; s < b->slot_pair.size(); ++s) {
+        const int32_t p = b->slot_pair[s];
+        if (b->status_h[s] != 0) {
+            results[p] = unsettled_code(b, b->status_h[s]);
+            continue;
+        }
+        const int64_t cost = b->cost_h[s];
+        results[p] = capped && cost > b->options.max_cost ? DINARA_ABOVE_MAX : cost;
+    }
+    return 0;
+}
+
+int64_t dinara_batch_alignments(dinara_batch *b, dinara_alignment *alignments, int64_t *statuses) {
+    if (!b || b->kind != DINARA_BATCH_ALIGNMENTS || !b->ran) return DINARA_INVALID_MODE;
+    if (const int64_t code = fetch(b)) return code;
+    const bool capped = b->options.max_cost >= 0;
+    for (int64_t i = 0; i < b->pairs; ++i) {
+        statuses[i] = b->code[i];
+        if (b->code[i] != 0) {
+            alignments[i].cigar = nullptr;
+            alignments[i].cigar_length = 0;
+        }
+    }
+    std::atomic<bool> oom{false};
+    parallel_for((int64_t)b->slot_pair.size(), b->threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t s = lo; s < hi; ++s) {
+            const int32_t p = b->slot_pair[s];
+            dinara_alignment &out = alignments[p];
+            out.cigar = nullptr;
+            out.cigar_length = 0;
+            if (b->status_h[s] != 0) {
+                statuses[p] = unsettled_code(b, b->status_h[s]);
+                continue;
+            }
+            const int64_t cost = b->cost_h[s];
+            if (capped && cost > b->options.max_cost) {
+                statuses[p] = DINARA_ABOVE_MAX;
+                continue;
+            }
+            const int64_t length = b->cigar_length_h[s];
+            char *text = static_cast<char *>(std::malloc(length + 1));
+            if (!text) {
+                statuses[p] = DINARA_OUT_OF_MEMORY;
+                oom.store(true);
+                continue;
+            }
+            std::memcpy(text, b->cigars_h.data() + b->cigar_offset_h[s], length);
+            text[length] = 0;
+            out.cost = cost;
+            out.score = -cost;
+            out.reference_start = 0;
+            out.reference_end = b->reference_length[p];
+            out.query_start = 0;
+            out.query_end = b->query_length[p];
+            out.cigar = text;
+            out.cigar_length = length;
+            statuses[p] = 0;
+        }
+    });
+    return 0;
+}
+
+void dinara_batch_free(dinara_batch *batch) {
+    if (!batch) return;
+    {
+        DeviceGuard guard(batch->ctx);
+        delete batch;
+    }
+}
+
+double dinara_batch_stat(dinara_batch *b, int64_t what) {
+    if (!b) return 0;
+    switch (what) {
+        case 0: return b->run_ms;
+        case 1: return b->sweep_ms;
+        case 2: return b->launches;
+        case 3: return b->cells;
+        case 4: return b->first_proven;
+        default: break;
+    }
+    if (what >= 100 && what < 500 && what % 100 < CLASSES) {
+        const int c = (int)(what % 100);
+        switch (what / 100) {
+            case 1: return b->class_ms[c];
+            case 2: return b->class_launches[c];
+            case 3: return b->class_cells[c];
+            case 4: return b->class_pairs[c];
+        }
+    }
+    switch (what) {
+        default: return 0;
+    }
+}
+
+int64_t dinara_gpu_distances(int64_t gpu, int64_t pairs, const char *const *references,
+                             const int64_t *reference_lengths, const char *const *queries,
+                             const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                             const dinara_options *options, int64_t threads, int64_t *results) {
+    ParsedCosts parsed = parse_costs(costs);
+    if (parsed.code) return parsed.code;
+    if (const int64_t code = check_mode(mode)) return code;
+    if (pairs <= 0) return 0;
+    int64_t status = 0;
+    dinara_batch *b = batch_new(gpu, DINARA_BATCH_DISTANCES, pairs, references, reference_lengths, queries,
+                                query_lengths, costs, mode, options, threads, &status);
+    if (!b) return status;
+    int64_t code = run(b);
+    if (code == 0) code = dinara_batch_distances(b, results);
+    dinara_batch_free(b);
+    return code;
+}
+
+int64_t dinara_gpu_alignments(int64_t gpu, int64_t pairs, const char *const *references,
+                              const int64_t *reference_lengths, const char *const *queries,
+                              const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                              const dinara_options *options, int64_t threads, dinara_alignment *alignments,
+                              int64_t *statuses) {
+    ParsedCosts parsed = parse_costs(costs);
+    if (parsed.code) return parsed.code;
+    if (const int64_t code = check_mode(mode)) return code;
+    if (pairs <= 0) return 0;
+    int64_t status = 0;
+    dinara_batch *b = batch_new(gpu, DINARA_BATCH_ALIGNMENTS, pairs, references, reference_lengths, queries,
+                                query_lengths, costs, mode, options, threads, &status);
+    if (!b) return status;
+    int64_t code = run(b);
+    if (code == 0) code = dinara_batch_alignments(b, alignments, statuses);
+    dinara_batch_free(b);
+    return code;
+}
+
+int64_t dinara_distances(int64_t pairs, const char *const *references, const int64_t *reference_lengths,
+                         const char *const *queries, const int64_t *query_lengths, const dinara_costs *costs,
+                         const dinara_mode *mode, const dinara_options *options, int64_t threads, int64_t *results) {
+    return dinara_gpu_distances(0, pairs, references, reference_lengths, queries, query_lengths, costs, mode, options,
+                                threads, results);
+}
+
+int64_t dinara_alignments(int64_t pairs, const char *const *references, const int64_t *reference_lengths,
+                          const char *const *queries, const int64_t *query_lengths, const dinara_costs *costs,
+                          const dinara_mode *mode, const dinara_options *options, int64_t threads,
+                          dinara_alignment *alignments, int64_t *statuses) {
+    return dinara_gpu_alignments(0, pairs, references, reference_lengths, queries, query_lengths, costs, mode,
+                                 options, threads, alignments, statuses);
+}
+
+}  // extern "C"

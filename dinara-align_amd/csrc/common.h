@@ -1,0 +1,84 @@
+// Shared device/host definitions of the MI355X alignment kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dinara {
+
+// A cell no path inside the band reaches. Every real cost stays below REAL_LIMIT (the host refuses pairs
+// whose dearest path could pass it), so INF plus a few moves never wraps and never reads as real.
+constexpr int INF = 1 << 28;
+constexpr int REAL_LIMIT = 1 << 26;
+
+// Gap-affine costs in the sweep's terms (reference lanes.mojo:187-259, modes.mojo:35-195): a deletion, a
+// reference letter alone, of k letters costs del_open + k * del_ext; an insertion ins_open + k * ins_ext.
+struct CostParams {
+    int mismatch;
+    int del_open, del_ext, ins_open, ins_ext;
+    int del_first, ins_first;  // a gap's first letter: opening plus extension
+};
+
+// Which pairs a launch takes and where their letters lie. The tape holds every sequence 2 or 8 bits a
+// letter, 16 or 4 to a 32-bit word, transposed by groups of 64 slots: slot s (the s-th pair in length
+// order) has word w of its reference at tape[group_base[s / 64] + w * 64 + s % 64], its query words after
+// group_ref_words[s / 64] reference words.
+struct Tape {
+    const uint32_t *words;
+    const int64_t *group_base;
+    const int32_t *group_ref_words;
+    const int32_t *rows;  // reference letters per slot
+    const int32_t *cols;  // query letters per slot
+    int bits;
+};
+
+// The sweep classes: three band widths one thread per pair holds, then wave-per-pair bands of 64 * Q.
+constexpr int CLASS_LANE16 = 0;
+constexpr int CLASS_LANE32 = 1;
+constexpr int CLASS_LANE64 = 2;
+constexpr int CLASS_WAVE_FIRST = 3;  // Q = 2, 4, 8, 16, 32 for classes 3..7
+constexpr int CLASS_WAVE_Q1 = 8;     // Q = 1: a 64-diagonal band under a user band, which the lanes refuse
+constexpr int CLASSES = 9;
+constexpr int CLASS_TOO_WIDE = -1;
+
+// Everything a sweep launch reads and writes.
+struct SweepArgs {
+    Tape tape;
+    const int32_t *list;  // slots to sweep, `count` of them
+    int count;
+    int32_t *want_low;    // per slot: the lane diagonals (query less reference) the band must hold
+    int32_t *want_high;
+    CostParams costs;
+    int strict;           // 1 for alignments: no path off the band may cost as little
+    int64_t cap;          // the cost cap, or a value past every cost
+    int user_band;        // 1 when diagonals outside [user_low, user_high] (lane terms) are barred
+    int user_low, user_high;
+    int eqx;
+    int forward_cigar;    // 1 when the tape holds both sequences reversed (Ties.LEFT): moves come in order
+    int record;
+    // results per slot
+    int32_t *cost;
+    int32_t *status;      // 0 settled, 1 filed for a wider band
+    // the decisions of the launch, per 64-slot block of the list (lane kernels) or per pair (wave kernel)
+    uint32_t *flags;
+    const int64_t *flag_base;
+    // CIGARs: each slot's scratch run of cigar_capacity bytes from cigar_scratch_base, compacted into
+    // `cigars` at an offset taken from `cigar_used`
+    char *cigar_scratch;
+    const int64_t *cigar_scratch_base;
+    char *cigars;
+    unsigned long long *cigar_used;
+    int64_t *cigar_offset;
+    int32_t *cigar_length;
+    // unproven slots, filed by the class of the band that will prove them
+    int32_t *retry_lists;  // CLASSES lists of `retry_capacity` slots each
+    int32_t *retry_counts;
+    int retry_capacity;
+};
+
+// Host-side entry points of sweep.hip.
+int class_width(int cls);
+bool class_is_wave(int cls);
+void launch_sweep(int cls, const SweepArgs &a, size_t lds, hipStream_t stream);
+
+}  // namespace dinara

@@ -1,0 +1,47 @@
+// Device contexts and memory of the library: one stream per GPU, every call ordered on it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace dinara {
+
+struct DeviceContext {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr, sweep_start = nullptr, sweep_stop = nullptr;
+    size_t lds_limit = 0;  // dynamic LDS one workgroup may take
+    int compute_units = 0;
+    std::vector<hipEvent_t> pool;  // timing events, made as needed
+    hipEvent_t event(size_t i);
+};
+
+// The context of GPU `gpu`, made on first use; null when there is no such GPU.
+DeviceContext *device_context(int64_t gpu);
+int64_t device_count();
+
+// Makes the context's GPU current for the calling thread, and restores the previous one after.
+struct DeviceGuard {
+    int previous = -1;
+    explicit DeviceGuard(const DeviceContext *ctx);
+    ~DeviceGuard();
+};
+
+// Device memory that grows on demand and is freed with its owner.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t size = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer();
+    bool reserve(size_t bytes);                                          // at least `bytes`, contents dropped
+    bool upload(DeviceContext *ctx, const void *host, size_t bytes);     // reserve, copy, wait
+    bool write(DeviceContext *ctx, const void *host, size_t bytes);      // copy into what is there, wait
+    bool read(DeviceContext *ctx, void *host, size_t bytes) const;       // copy out after the stream's work
+};
+
+}  // namespace dinara

@@ -1,0 +1,184 @@
+/*
+ * dinara-align on MI355X: the C ABI of libdinara_hip.so.
+ *
+ * The batch entry points run on an AMD Instinct MI355X (gfx950) through hand-written HIP kernels. Their
+ * struct layouts, argument meanings and result codes are those of the reference library's C header, so a
+ * caller of the reference's batch calls links against this library unchanged; the device entry points
+ * (dinara_gpu_*, dinara_batch_*) are new symbols, so the header only grows.
+ *
+ * Sequences are bytes, each matching only itself, save 0xFE and 0xFF, which are refused. The first
+ * sequence of a pair is the reference, the second the query: a CIGAR's `=` is a match, `X` a
+ * substitution (`M` for either without eqx), `D` a reference letter alone, `I` a query letter alone.
+ *
+ * What the device serves: global alignment under unit, linear or one-piece gap-affine costs, deletions
+ * optionally priced apart from insertions, any band and any cost cap; and, under a substitution table,
+ * global and local scores and alignments. Other modes (free ends, extension, two gap pieces) return
+ * DINARA_INVALID_MODE for the whole batch: this library holds no CPU fallback.
+ */
+#ifndef DINARA_H
+#define DINARA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-pair and batch-wide result codes: reference c/dinara.h:35-49. */
+#define DINARA_UNSUPPORTED_SYMBOLS (-1) /* a 0xFE or 0xFF byte, or a letter outside a Scoring's alphabet */
+#define DINARA_OUT_OF_MEMORY (-2)       /* device or host memory ran out */
+#define DINARA_INVALID_COSTS (-3)       /* costs no alignment can be searched by */
+#define DINARA_ABOVE_MAX (-4)           /* every alignment costs more than max_cost */
+#define DINARA_OUTSIDE_BAND (-5)        /* no alignment stays inside the band */
+#define DINARA_INVALID_MODE (-6)        /* a mode this library does not serve */
+#define DINARA_INVALID_LENGTH (-7)      /* a sequence's length below zero */
+#define DINARA_NO_DEVICE (-8)           /* new: no usable GPU, or the GPU id is out of range */
+
+/* Reference c/dinara.h:59-69. A null pointer is unit costs {1, 0, 1, -1, 0}. */
+typedef struct {
+    int64_t mismatch;            /* a substitution's cost */
+    int64_t opening;             /* a gap's cost before its letters */
+    int64_t extension;           /* each gapped letter's cost */
+    int64_t opening2;            /* the second piece's opening; negative for one piece */
+    int64_t extension2;          /* the second piece's extension */
+    int64_t deletion_opening;    /* a deletion's opening, with deletion_extension above zero */
+    int64_t deletion_extension;  /* a deletion's extension; zero for deletions costing as insertions */
+    int64_t deletion_opening2;   /* a deletion's second opening; negative for one piece */
+    int64_t deletion_extension2; /* a deletion's second extension */
+} dinara_costs;
+
+#define DINARA_ENDS_FREE 0
+#define DINARA_EXTENSION 1
+#define DINARA_LOCAL 2
+#define DINARA_ALL INT64_MAX
+
+/* Reference c/dinara.h:91-101. A null pointer is a global alignment. */
+typedef struct {
+    int64_t kind;
+    int64_t reference_start;
+    int64_t reference_end;
+    int64_t query_start;
+    int64_t query_end;
+    int64_t match_score;
+    int64_t anchor;
+    int64_t zdrop;
+    int64_t end_bonus;
+} dinara_mode;
+
+/* Reference c/dinara.h:115-122. A band's diagonal is reference letters less query letters; INT64_MIN and
+ * INT64_MAX for no band. A null pointer is no band, no cap, eqx, indels left, default memory. */
+typedef struct {
+    int64_t band_low;
+    int64_t band_high;
+    int64_t max_cost;   /* the cap when zero or more; negative for none */
+    int64_t eqx;        /* nonzero writes `=` and `X`, zero `M` */
+    int64_t right_ties; /* nonzero places tied indels right (WFA2-lib's rule), zero left */
+    int64_t max_memory; /* accepted for compatibility; the device traces every pair whole */
+} dinara_options;
+
+/* Reference c/dinara.h:129-138. */
+typedef struct {
+    int64_t cost;
+    int64_t score;
+    int64_t reference_start;
+    int64_t reference_end;
+    int64_t query_start;
+    int64_t query_end;
+    char *cigar;          /* NUL-terminated; free with dinara_free */
+    int64_t cigar_length; /* its bytes, the NUL not counted */
+} dinara_alignment;
+
+/* Frees a CIGAR a batch returned; null is nothing to free. Replaces reference c/dinara.h:207. */
+void dinara_free(char *cigar);
+
+/*
+ * A batch on GPU 0: pair i is references[i] (reference_lengths[i] bytes) against queries[i]. Zero, or a
+ * code failing the whole batch (costs or a mode no pair can take), and then nothing written. `threads`
+ * above zero packs the batch on that many host threads; zero or less packs on the caller's thread.
+ * Replace reference c/dinara.h:219-222 and :225-228 (same signatures, same result codes).
+ */
+int64_t dinara_distances(int64_t pairs, const char *const *references, const int64_t *reference_lengths,
+                         const char *const *queries, const int64_t *query_lengths, const dinara_costs *costs,
+                         const dinara_mode *mode, const dinara_options *options, int64_t threads, int64_t *results);
+
+int64_t dinara_alignments(int64_t pairs, const char *const *references, const int64_t *reference_lengths,
+                          const char *const *queries, const int64_t *query_lengths, const dinara_costs *costs,
+                          const dinara_mode *mode, const dinara_options *options, int64_t threads,
+                          dinara_alignment *alignments, int64_t *statuses);
+
+/* New: how many GPUs the library sees. No counterpart in the reference (its Mojo Placement.on_gpu,
+ * reference common.mojo:130-160, names the device instead). */
+int64_t dinara_device_count(void);
+
+/* New: dinara_distances / dinara_alignments on GPU `gpu`, mirroring the reference's Mojo
+ * distances/alignments(..., placement=Placement.on_gpu(gpu, threads)), reference api.mojo:895-903,1216-1225. */
+int64_t dinara_gpu_distances(int64_t gpu, int64_t pairs, const char *const *references,
+                             const int64_t *reference_lengths, const char *const *queries,
+                             const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                             const dinara_options *options, int64_t threads, int64_t *results);
+
+int64_t dinara_gpu_alignments(int64_t gpu, int64_t pairs, const char *const *references,
+                              const int64_t *reference_lengths, const char *const *queries,
+                              const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                              const dinara_options *options, int64_t threads, dinara_alignment *alignments,
+                              int64_t *statuses);
+
+/* New: a substitution table, as the reference's Mojo/Python `Scoring` (reference scoring.mojo:126-180,
+ * python/dinara_align/__init__.py:272-309): `letters` letters of `alphabet`, `table[a * letters + b]` what
+ * reference letter a against query letter b scores (-128..127), a gap of k letters `opening + k * extension`,
+ * both zero or less. At most 32 letters. */
+typedef struct {
+    const char *alphabet;
+    int64_t letters;
+    const int8_t *table;
+    int64_t opening;
+    int64_t extension;
+} dinara_scoring;
+
+/* New: every pair's best score under `scoring` (global, or DINARA_LOCAL with match_score zero), mirroring
+ * the reference's Mojo api.scores (reference api.mojo:843-892). Per-pair codes in results as for distances. */
+int64_t dinara_gpu_scores(int64_t gpu, int64_t pairs, const char *const *references,
+                          const int64_t *reference_lengths, const char *const *queries, const int64_t *query_lengths,
+                          const dinara_scoring *scoring, const dinara_mode *mode, int64_t threads, int64_t *scores);
+
+/* New: every pair's alignment under `scoring`, mirroring the reference's Mojo api.alignments with a Scoring
+ * (reference api.mojo:895-1001): Gotoh's walk picks among equal alignments; a local alignment's spans are
+ * placed at the last place its letters occur (reference scoring.mojo:724-745). cost = -score. */
+int64_t dinara_gpu_scoring_alignments(int64_t gpu, int64_t pairs, const char *const *references,
+                                      const int64_t *reference_lengths, const char *const *queries,
+                                      const int64_t *query_lengths, const dinara_scoring *scoring,
+                                      const dinara_mode *mode, int64_t eqx, int64_t threads,
+                                      dinara_alignment *alignments, int64_t *statuses);
+
+/*
+ * New: a batch kept resident on the device, for callers that run the same pairs more than once (and for
+ * timing the device path alone). dinara_batch_new validates, packs and uploads; dinara_batch_run runs the
+ * kernels with every input already in HBM and waits for them; dinara_batch_distances / _alignments copy the
+ * last run's results out, in the caller's order, as dinara_distances / dinara_alignments write them.
+ * `kind` is DINARA_BATCH_DISTANCES or DINARA_BATCH_ALIGNMENTS. No counterpart in the reference.
+ */
+typedef struct dinara_batch dinara_batch;
+#define DINARA_BATCH_DISTANCES 0
+#define DINARA_BATCH_ALIGNMENTS 1
+
+dinara_batch *dinara_batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *const *references,
+                               const int64_t *reference_lengths, const char *const *queries,
+                               const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                               const dinara_options *options, int64_t threads, int64_t *status);
+int64_t dinara_batch_run(dinara_batch *batch);
+int64_t dinara_batch_distances(dinara_batch *batch, int64_t *results);
+int64_t dinara_batch_alignments(dinara_batch *batch, dinara_alignment *alignments, int64_t *statuses);
+void dinara_batch_free(dinara_batch *batch);
+
+/* New: what the last dinara_batch_run spent, read from HIP events on the library's stream: `what` 0 the
+ * whole run (ms), 1 the banded sweep-and-trace kernels (ms), 2 the number of sweep launches, 3 the cells the
+ * sweeps computed, 4 the pairs the first band proved; for band class c (0-2 one thread a pair, 16/32/64
+ * diagonals; 3-7 one wave a pair, 128..2048; 8 one wave, 64 under a user band) 100+c its kernels' time (ms),
+ * 200+c its launches, 300+c its cells, 400+c the pairs it swept. For roofline reporting. */
+double dinara_batch_stat(dinara_batch *batch, int64_t what);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
