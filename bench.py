@@ -157,7 +157,7 @@ def main() -> int:
         if handle.dinara_batch_run(batch) != 0:
             print("[bench] warm-up run failed", file=sys.stderr)
             return 1
-    walls, device_ms, class_ms, class_launches, class_pairs, class_cells = [], [], None, None, None, None
+    walls, device_ms, walk_ms, class_ms, class_launches, class_pairs = [], [], [], None, None, None
     for step in range(args.steps):
         t0 = time.perf_counter()
         code = handle.dinara_batch_run(batch)
@@ -166,10 +166,10 @@ def main() -> int:
             print(f"[bench] run failed: {code}", file=sys.stderr)
             return 1
         device_ms.append(handle.dinara_batch_stat(batch, 0))
+        walk_ms.append(handle.dinara_batch_stat(batch, 5))
         ms = np.array([handle.dinara_batch_stat(batch, 100 + c) for c in range(9)])
         class_ms = ms if class_ms is None else class_ms + ms
         class_launches = np.array([handle.dinara_batch_stat(batch, 200 + c) for c in range(9)])
-        class_cells = np.array([handle.dinara_batch_stat(batch, 300 + c) for c in range(9)])
         class_pairs = np.array([handle.dinara_batch_stat(batch, 400 + c) for c in range(9)])
     first_proven = handle.dinara_batch_stat(batch, 4)
     print(f"[bench] {args.steps} steps, {np.mean(walls) * 1e3:.2f} ms each", file=sys.stderr, flush=True)
@@ -198,7 +198,9 @@ def main() -> int:
     launch_ms = mean_class_ms[dominant] / launches
     algorithmic_ops = float(band_cells.sum()) * share * OPS_PER_CELL / launches
     achieved = algorithmic_ops / (launch_ms / 1e3) / 1e12
-    swept_ops = class_cells[dominant] * OPS_PER_CELL / launches / (launch_ms / 1e3) / 1e12
+    width = CLASS_WIDTHS[dominant]
+    swept_cells = float(np.mean(n)) * width * class_pairs[dominant] / launches
+    swept_ops = swept_cells * OPS_PER_CELL / (launch_ms / 1e3) / 1e12
     roofline = {
         "bound": "valu",
         "kernel": CLASS_NAMES[dominant],
@@ -267,8 +269,9 @@ def main() -> int:
         "bit_exact_vs_cpu": exact,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels_ms_per_step": {CLASS_NAMES[c]: round(float(mean_class_ms[c]), 4) for c in range(9)
-                                if mean_class_ms[c] > 0},
+        "kernels_ms_per_step": {**{CLASS_NAMES[c]: round(float(mean_class_ms[c]), 4) for c in range(9)
+                                   if mean_class_ms[c] > 0}, "walk_lanes": round(float(np.mean(walk_ms)), 4)},
+        "pairs_per_class": {CLASS_NAMES[c]: int(class_pairs[c]) for c in range(9) if class_pairs[c] > 0},
         "pack_upload_s": round(pack_s, 3),
     }
     print(json.dumps(line))

@@ -165,19 +165,22 @@ struct dinara_batch {
     int bits = 2;
     size_t lds_limit = 0;
     DeviceContext *ctx = nullptr;
+    int max_letters = 0;                  // the most letters of a pair, both sequences
+    int32_t max_rows = 0;
+    std::vector<int64_t> first_offset;    // where each class's first list starts in first_lists_d
     // device memory
     DeviceBuffer tape, group_base, group_ref_words, rows_d, cols_d, want_low, want_high, first_low_d,
-        first_high_d, cost, status, cigar_scratch, cigar_scratch_base, cigars, cigar_used, cigar_offset,
-        cigar_length, retry_lists, retry_counts, list_d, flag_base_d, flags;
+        first_high_d, cost, status, walk_low, walk_class, cigar_scratch, cigar_scratch_base, cigars, cigar_used,
+        cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, lane_flags, lane_base, arena,
+        arena_used;
     // the last run's results, by slot
     std::vector<int32_t> cost_h, status_h, cigar_length_h;
     std::vector<int64_t> cigar_offset_h;
     std::vector<char> cigars_h;
     bool ran = false;
     // statistics of the last run
-    double run_ms = 0, sweep_ms = 0, launches = 0, cells = 0, first_proven = 0;
-    double class_ms[CLASSES] = {}, class_launches[CLASSES] = {}, class_cells[CLASSES] = {},
-           class_pairs[CLASSES] = {};
+    double run_ms = 0, sweep_ms = 0, walk_ms = 0, launches = 0, first_proven = 0;
+    double class_ms[CLASSES] = {}, class_launches[CLASSES] = {}, class_pairs[CLASSES] = {};
 };
 
 namespace {
@@ -195,6 +198,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     });
     const int64_t slots = (int64_t)order.size();
     b->slot_pair = order;
+    b->first_offset.assign(CLASSES + 1, 0);
     if (slots == 0) return 0;  // every pair answered by its own code
     b->rows.resize(slots);
     b->cols.resize(slots);
@@ -297,6 +301,33 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         if (b->kind == DINARA_BATCH_ALIGNMENTS) scratch += 2 * ((int64_t)n + m) + 2;
     }
     b->scratch_base[slots] = scratch;
+    b->max_rows = 0;
+    b->max_letters = 0;
+    for (int64_t s = 0; s < slots; ++s) {
+        b->max_rows = std::max(b->max_rows, b->rows[s]);
+        b->max_letters = std::max<int>(b->max_letters, b->rows[s] + b->cols[s]);
+    }
+    // Each lane class's decisions for every slot, rows of W/2 bytes one after another.
+    std::vector<int64_t> lane_base;
+    int64_t lane_bytes = 0;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS) {
+        lane_base.resize(3 * slots);
+        for (int c = 0; c < 3; ++c) {
+            const int64_t row_bytes = class_width(c) / 2;
+            for (int64_t s = 0; s < slots; ++s) {
+                lane_base[c * slots + s] = lane_bytes;
+                lane_bytes += (int64_t)b->rows[s] * row_bytes;
+            }
+            lane_bytes = (lane_bytes + 255) & ~int64_t(255);
+        }
+    }
+    std::vector<int32_t> first_all;
+    b->first_offset.assign(CLASSES + 1, 0);
+    for (int c = 0; c < CLASSES; ++c) {
+        b->first_offset[c] = (int64_t)first_all.size();
+        first_all.insert(first_all.end(), b->first_lists[c].begin(), b->first_lists[c].end());
+    }
+    b->first_offset[CLASSES] = (int64_t)first_all.size();
 
     // Upload.
     DeviceContext *ctx = b->ctx;
@@ -309,9 +340,16 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         !b->first_high_d.upload(ctx, b->first_high.data(), std::max<int64_t>(slots, 1) * 4) ||
         !b->want_low.reserve(std::max<int64_t>(slots, 1) * 4) || !b->want_high.reserve(std::max<int64_t>(slots, 1) * 4) ||
         !b->cost.reserve(std::max<int64_t>(slots, 1) * 4) || !b->status.reserve(std::max<int64_t>(slots, 1) * 4) ||
-        !b->retry_lists.reserve((size_t)CLASSES * std::max<int64_t>(slots, 1) * 4) ||
-        !b->retry_counts.reserve(CLASSES * 4) || !b->list_d.reserve(std::max<int64_t>(slots, 1) * 4 + 64) ||
-        !b->flag_base_d.reserve(std::max<int64_t>(slots, 1) * 8 + 64))
+        !b->walk_low.reserve(std::max<int64_t>(slots, 1) * 4) ||
+        !b->walk_class.reserve(std::max<int64_t>(slots, 1) * 4) ||
+        !b->retry_lists.reserve((size_t)2 * CLASSES * std::max<int64_t>(slots, 1) * 4) ||
+        !b->retry_counts.reserve(2 * CLASSES * 4) ||
+        !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4) ||
+        !b->arena_used.reserve(8))
+        return DINARA_OUT_OF_MEMORY;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS &&
+        (!b->lane_flags.reserve(std::max<int64_t>(lane_bytes, 256)) ||
+         !b->lane_base.upload(ctx, lane_base.data(), lane_base.size() * 8)))
         return DINARA_OUT_OF_MEMORY;
     if (b->kind == DINARA_BATCH_ALIGNMENTS) {
         if (!b->cigar_scratch.reserve(std::max<int64_t>(scratch, 1)) ||
@@ -332,6 +370,7 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.tape.rows = static_cast<const int32_t *>(b->rows_d.ptr);
     a.tape.cols = static_cast<const int32_t *>(b->cols_d.ptr);
     a.tape.bits = b->bits;
+    a.slots = (int)b->slot_pair.size();
     a.want_low = static_cast<int32_t *>(b->want_low.ptr);
     a.want_high = static_cast<int32_t *>(b->want_high.ptr);
     a.costs = b->costs;
@@ -343,86 +382,66 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.eqx = b->options.eqx;
     a.forward_cigar = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
     a.record = b->kind == DINARA_BATCH_ALIGNMENTS;
+    a.lds_limit = (int)b->lds_limit;
     a.cost = static_cast<int32_t *>(b->cost.ptr);
     a.status = static_cast<int32_t *>(b->status.ptr);
+    a.walk_low = static_cast<int32_t *>(b->walk_low.ptr);
+    a.walk_class = static_cast<int32_t *>(b->walk_class.ptr);
+    a.lane_flags = static_cast<uint8_t *>(b->lane_flags.ptr);
+    a.lane_base = static_cast<const int64_t *>(b->lane_base.ptr);
+    a.arena = static_cast<uint8_t *>(b->arena.ptr);
+    a.arena_used = static_cast<unsigned long long *>(b->arena_used.ptr);
     a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
     a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
     a.cigars = static_cast<char *>(b->cigars.ptr);
     a.cigar_used = static_cast<unsigned long long *>(b->cigar_used.ptr);
     a.cigar_offset = static_cast<int64_t *>(b->cigar_offset.ptr);
     a.cigar_length = static_cast<int32_t *>(b->cigar_length.ptr);
-    a.retry_lists = static_cast<int32_t *>(b->retry_lists.ptr);
-    a.retry_counts = static_cast<int32_t *>(b->retry_counts.ptr);
     a.retry_capacity = (int)std::max<size_t>(b->slot_pair.size(), 1);
     return a;
 }
 
-constexpr size_t FLAG_BUDGET = size_t(8) << 30;  // decisions one launch may keep
+constexpr size_t ARENA_BUDGET = size_t(16) << 30;  // decisions one wave launch may keep
 
-// Sweeps one class over `list`, in launches whose decisions fit the budget.
-int64_t sweep_class(dinara_batch *b, int cls, const std::vector<int32_t> &list) {
+// Launches class `cls` over `count` slots listed at `list` (device memory), filing retries into `retry_lists` /
+// `retry_counts`; wave launches are cut so each one's decisions fit the arena.
+int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t count, int32_t *retry_lists,
+                     int32_t *retry_counts) {
     DeviceContext *ctx = b->ctx;
     SweepArgs a = sweep_args(b);
+    a.cls = cls;
+    a.retry_lists = retry_lists;
+    a.retry_counts = retry_counts;
     const bool wave = class_is_wave(cls);
-    const int width = class_width(cls);
-    const bool record = a.record;
-    size_t start = 0;
-    while (start < list.size()) {
-        // Take blocks (64 pairs, lanes) or pairs (waves) while their decisions fit.
-        std::vector<int64_t> bases;
-        size_t stop = start;
-        size_t bytes = 0;
-        size_t lds = 0;
-        const size_t step = wave ? 1 : 64;
-        while (stop < list.size()) {
-            const size_t next = std::min(list.size(), stop + step);
-            size_t need = 0;
-            if (record) {
-                if (wave) {
-                    need = ((size_t)b->rows[list[stop]] * width + 3) & ~size_t(3);
-                } else {
-                    int32_t rows = 0;
-                    for (size_t i = stop; i < next; ++i) rows = std::max(rows, b->rows[list[i]]);
-                    need = (size_t)rows * (width / 8) * 64 * 4;
-                }
-            }
-            if (stop > start && bytes + need > FLAG_BUDGET) break;
-            bases.push_back(wave ? (int64_t)bytes : (int64_t)(bytes / 4));
-            bytes += need;
-            if (wave) lds = std::max(lds, (size_t)b->rows[list[stop]] + (size_t)b->cols[list[stop]]);
-            stop = next;
-        }
-        const int count = (int)(stop - start);
-        if (wave && lds > b->lds_limit) return DINARA_OUT_OF_MEMORY;
-        if (!b->list_d.write(ctx, list.data() + start, count * 4) ||
-            !b->flag_base_d.write(ctx, bases.data(), bases.size() * 8))
-            return DINARA_OUT_OF_MEMORY;
-        if (record && !b->flags.reserve(std::max<size_t>(bytes, 4))) return DINARA_OUT_OF_MEMORY;
-        a.list = static_cast<const int32_t *>(b->list_d.ptr);
-        a.count = count;
-        a.flags = static_cast<uint32_t *>(b->flags.ptr);
-        a.flag_base = static_cast<const int64_t *>(b->flag_base_d.ptr);
-        launch_sweep(cls, a, wave ? std::max<size_t>(lds, 4) : 0, ctx->stream);
+    const size_t lds = wave ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit) : 0;
+    int64_t chunk = count;
+    if (wave && a.record) {
+        const size_t per_pair = ((size_t)b->max_rows * class_width(cls) + 15) & ~size_t(15);
+        chunk = std::max<int64_t>(1, (int64_t)std::min<size_t>(count, ARENA_BUDGET / std::max<size_t>(per_pair, 1)));
+        if (!b->arena.reserve(std::max<size_t>(per_pair * chunk, 16))) return DINARA_OUT_OF_MEMORY;
+        a.arena = static_cast<uint8_t *>(b->arena.ptr);
+    }
+    for (int64_t start = 0; start < count; start += chunk) {
+        a.list = list + start;
+        a.count = (int)std::min(chunk, count - start);
+        if (wave && a.record) hipMemsetAsync(b->arena_used.ptr, 0, 8, ctx->stream);
+        launch_sweep(a, lds, ctx->stream);
         if (hipGetLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
         b->launches += 1;
         b->class_launches[cls] += 1;
-        b->class_pairs[cls] += count;
-        for (size_t i = start; i < stop; ++i) {
-            b->cells += (double)b->rows[list[i]] * width;
-            b->class_cells[cls] += (double)b->rows[list[i]] * width;
-        }
-        start = stop;
+        b->class_pairs[cls] += a.count;
     }
     return 0;
 }
 
 int64_t run(dinara_batch *b) {
     DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
     DeviceGuard guard(ctx);
     const int64_t slots = (int64_t)b->slot_pair.size();
-    b->launches = b->cells = b->first_proven = 0;
-    b->run_ms = b->sweep_ms = 0;
-    for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_cells[c] = b->class_pairs[c] = 0;
+    b->launches = b->first_proven = 0;
+    b->run_ms = b->sweep_ms = b->walk_ms = 0;
+    for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_pairs[c] = 0;
     hipEventRecord(ctx->start, ctx->stream);
     if (slots > 0) {
         hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream);
@@ -430,54 +449,65 @@ int64_t run(dinara_batch *b) {
         if (b->kind == DINARA_BATCH_ALIGNMENTS) hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream);
         hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream);
     }
-    std::vector<int32_t> lists[CLASSES];
-    for (int c = 0; c < CLASSES; ++c) lists[c] = b->first_lists[c];
-    bool first = true;
-    for (int round = 0; slots > 0; ++round) {
-        bool any = false;
-        for (int c = 0; c < CLASSES; ++c) any = any || !lists[c].empty();
-        if (!any || round > 64) break;
-        hipMemsetAsync(b->retry_counts.ptr, 0, CLASSES * 4, ctx->stream);
+    // Round 0 sweeps the first lists (uploaded once); each later round sweeps the lists the round before filed,
+    // kept on the device, two sets taking turns. Only the counts come back to the host.
+    int64_t counts[CLASSES] = {};
+    if (slots > 0)
+        for (int c = 0; c < CLASSES; ++c) counts[c] = b->first_offset[c + 1] - b->first_offset[c];
+    int32_t *lists = static_cast<int32_t *>(b->retry_lists.ptr);
+    int32_t *count_d = static_cast<int32_t *>(b->retry_counts.ptr);
+    const size_t capacity = (size_t)std::max<int64_t>(slots, 1);
+    for (int round = 0; slots > 0 && round < 64; ++round) {
+        int64_t pending = 0;
+        for (int c = 0; c < CLASSES; ++c) pending += counts[c];
+        if (pending == 0) break;
+        int32_t *out_lists = lists + (size_t)(round % 2) * CLASSES * capacity;
+        int32_t *out_counts = count_d + (round % 2) * CLASSES;
+        const int32_t *in_lists = lists + (size_t)((round + 1) % 2) * CLASSES * capacity;
+        hipMemsetAsync(out_counts, 0, CLASSES * 4, ctx->stream);
         hipEventRecord(ctx->sweep_start, ctx->stream);
-        // Each class's launches between two events of their own, for the per-kernel time.
         for (int c = 0; c < CLASSES; ++c) {
-            if (lists[c].empty()) continue;
+            if (counts[c] == 0) continue;
+            const int32_t *list = round == 0 ? static_cast<const int32_t *>(b->first_lists_d.ptr) + b->first_offset[c]
+                                             : in_lists + (size_t)c * capacity;
+            // Each class's launches between two events of their own, for the per-kernel time.
             hipEventRecord(ctx->event(2 * c), ctx->stream);
-            const int64_t code = sweep_class(b, c, lists[c]);
-            if (code) return code;
+            if (const int64_t code = launch_class(b, c, list, counts[c], out_lists, out_counts)) return code;
             hipEventRecord(ctx->event(2 * c + 1), ctx->stream);
         }
         hipEventRecord(ctx->sweep_stop, ctx->stream);
-        int32_t counts[CLASSES];
-        if (hipMemcpyAsync(counts, b->retry_counts.ptr, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream) !=
-                hipSuccess ||
+        int32_t filed[CLASSES];
+        if (hipMemcpyAsync(filed, out_counts, sizeof(filed), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             return DINARA_OUT_OF_MEMORY;
         float ms = 0;
         hipEventElapsedTime(&ms, ctx->sweep_start, ctx->sweep_stop);
         b->sweep_ms += ms;
-        for (int c = 0; c < CLASSES; ++c) {
-            if (lists[c].empty()) continue;
-            float class_ms = 0;
-            hipEventElapsedTime(&class_ms, ctx->event(2 * c), ctx->event(2 * c + 1));
-            b->class_ms[c] += class_ms;
-        }
         int64_t retried = 0;
         for (int c = 0; c < CLASSES; ++c) {
-            lists[c].resize(counts[c]);
-            retried += counts[c];
-            if (counts[c] > 0)
-                hipMemcpyAsync(lists[c].data(), static_cast<int32_t *>(b->retry_lists.ptr) + (size_t)c * std::max<int64_t>(slots, 1),
-                               counts[c] * 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (counts[c] > 0) {
+                float class_ms = 0;
+                hipEventElapsedTime(&class_ms, ctx->event(2 * c), ctx->event(2 * c + 1));
+                b->class_ms[c] += class_ms;
+            }
+            counts[c] = filed[c];
+            retried += filed[c];
         }
-        if (first) b->first_proven = (double)(slots - retried);
-        first = false;
-        if (retried > 0 && hipStreamSynchronize(ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
-        // A lane launch's list holds a block's worth of slots in length order; keep retries so too.
-        for (int c = 0; c < CLASSES; ++c) std::sort(lists[c].begin(), lists[c].end());
+        if (round == 0) b->first_proven = (double)(slots - retried);
+    }
+    if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
+        SweepArgs a = sweep_args(b);
+        hipEventRecord(ctx->event(2 * CLASSES), ctx->stream);
+        launch_walk(a, ctx->stream);
+        hipEventRecord(ctx->event(2 * CLASSES + 1), ctx->stream);
     }
     hipEventRecord(ctx->stop, ctx->stream);
     if (hipEventSynchronize(ctx->stop) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
+        float walk_ms = 0;
+        hipEventElapsedTime(&walk_ms, ctx->event(2 * CLASSES), ctx->event(2 * CLASSES + 1));
+        b->walk_ms = walk_ms;
+    }
     float total = 0;
     hipEventElapsedTime(&total, ctx->start, ctx->stop);
     b->run_ms = total;
@@ -487,6 +517,7 @@ int64_t run(dinara_batch *b) {
 
 int64_t fetch(dinara_batch *b) {
     DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
     DeviceGuard guard(ctx);
     const int64_t slots = (int64_t)b->slot_pair.size();
     b->cost_h.resize(slots);
@@ -578,8 +609,12 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
             b->query_length[i] = std::max<int64_t>(m, 0);
         }
     });
-    DeviceGuard guard(ctx);
-    const int64_t code = prepare(b, refs, qrys);
+    int64_t code;
+    {
+        std::lock_guard<std::mutex> serial(ctx->lock);
+        DeviceGuard guard(ctx);
+        code = prepare(b, refs, qrys);
+    }
     if (code) {
         delete b;
         return fail(code);
@@ -692,8 +727,8 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 0: return b->run_ms;
         case 1: return b->sweep_ms;
         case 2: return b->launches;
-        case 3: return b->cells;
         case 4: return b->first_proven;
+        case 5: return b->walk_ms;
         default: break;
     }
     if (what >= 100 && what < 500 && what % 100 < CLASSES) {
@@ -701,7 +736,6 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         switch (what / 100) {
             case 1: return b->class_ms[c];
             case 2: return b->class_launches[c];
-            case 3: return b->class_cells[c];
             case 4: return b->class_pairs[c];
         }
     }

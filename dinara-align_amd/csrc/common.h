@@ -46,6 +46,8 @@ struct SweepArgs {
     Tape tape;
     const int32_t *list;  // slots to sweep, `count` of them
     int count;
+    int cls;              // the launch's band class
+    int slots;            // slots in the batch
     int32_t *want_low;    // per slot: the lane diagonals (query less reference) the band must hold
     int32_t *want_high;
     CostParams costs;
@@ -56,14 +58,20 @@ struct SweepArgs {
     int eqx;
     int forward_cigar;    // 1 when the tape holds both sequences reversed (Ties.LEFT): moves come in order
     int record;
+    int lds_limit;        // letters one wave kernel's LDS holds
     // results per slot
     int32_t *cost;
-    int32_t *status;      // 0 settled, 1 filed for a wider band
-    // the decisions of the launch, per 64-slot block of the list (lane kernels) or per pair (wave kernel)
-    uint32_t *flags;
-    const int64_t *flag_base;
-    // CIGARs: each slot's scratch run of cigar_capacity bytes from cigar_scratch_base, compacted into
-    // `cigars` at an offset taken from `cigar_used`
+    int32_t *status;      // 0 settled, 1 filed for a wider band, 2 wider than any band class
+    int32_t *walk_low;    // the settled band's lowest diagonal and its class, for the walk
+    int32_t *walk_class;
+    // decisions: the lane classes' per slot, W/2 bytes a row, at lane_flags + lane_base[cls * slots + slot];
+    // the wave classes' in an arena each launch reuses, W bytes a row, from arena_used
+    uint8_t *lane_flags;
+    const int64_t *lane_base;
+    uint8_t *arena;
+    unsigned long long *arena_used;
+    // CIGARs: each slot's scratch run of 2(n+m)+2 bytes from cigar_scratch_base, compacted into `cigars` at an
+    // offset taken from `cigar_used`
     char *cigar_scratch;
     const int64_t *cigar_scratch_base;
     char *cigars;
@@ -79,6 +87,7 @@ struct SweepArgs {
 // Host-side entry points of sweep.hip.
 int class_width(int cls);
 bool class_is_wave(int cls);
-void launch_sweep(int cls, const SweepArgs &a, size_t lds, hipStream_t stream);
+void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream);
+void launch_walk(const SweepArgs &a, hipStream_t stream);
 
 }  // namespace dinara

@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 namespace dinara {
@@ -17,6 +18,7 @@ struct DeviceContext {
     int compute_units = 0;
     std::vector<hipEvent_t> pool;  // timing events, made as needed
     hipEvent_t event(size_t i);
+    std::mutex lock;  // one caller's work on the stream at a time (the events are shared)
 };
 
 // The context of GPU `gpu`, made on first use; null when there is no such GPU.

@@ -88,11 +88,12 @@ __device__ int class_for(int need, int user_band) {
     return 64 * q >= need ? cls : CLASS_TOO_WIDE;
 }
 
-__device__ void file_retry(const SweepArgs &a, int slot, int want_low, int want_high) {
+__device__ void file_retry(const SweepArgs &a, int slot, int letters, int want_low, int want_high) {
     a.want_low[slot] = want_low;
     a.want_high[slot] = want_high;
     const int cls = class_for(want_high - want_low + 1, a.user_band);
-    if (cls == CLASS_TOO_WIDE) {
+    // A wave sweep stages both sequences in LDS: a pair they overflow has no class to go to.
+    if (cls == CLASS_TOO_WIDE || (cls >= CLASS_WAVE_FIRST && letters > a.lds_limit)) {
         a.status[slot] = 2;
         return;
     }
@@ -272,9 +273,11 @@ __global__ __launch_bounds__(64) void sweep_lane(SweepArgs a) {
         D[j] = INF;
     }
 
-    uint32_t *flags = nullptr;
-    if (RECORD) flags = a.flags + a.flag_base[t >> 6] + (t & 63);
+    // This pair's decisions, W/2 bytes a row, rows one after another.
+    uint8_t *flags = nullptr;
+    if (RECORD) flags = a.lane_flags + a.lane_base[(size_t)a.cls * a.slots + slot];
     constexpr int WORDS = W / 8;
+    uint32_t row_words[WORDS];
 
     // Rows whose band still holds column zero, or columns before it, take the border.
     const int border_rows = imin(n, -lo);
@@ -327,9 +330,20 @@ __global__ __launch_bounds__(64) void sweep_lane(SweepArgs a) {
             if (RECORD) {
                 word |= o.flag << (4 * (j % 8));
                 if (j % 8 == 7) {
-                    flags[((size_t)(r - 1) * WORDS + j / 8) * 64] = word;
+                    row_words[j / 8] = word;
                     word = 0;
                 }
+            }
+        }
+        if (RECORD) {
+            uint8_t *row = flags + (size_t)(r - 1) * (W / 2);
+            if constexpr (WORDS == 2) {
+                *reinterpret_cast<uint2 *>(row) = make_uint2(row_words[0], row_words[1]);
+            } else {
+#pragma unroll
+                for (int w = 0; w < WORDS; w += 4)
+                    *reinterpret_cast<uint4 *>(row + 4 * w) =
+                        make_uint4(row_words[w], row_words[w + 1], row_words[w + 2], row_words[w + 3]);
             }
         }
     }
@@ -340,28 +354,41 @@ __global__ __launch_bounds__(64) void sweep_lane(SweepArgs a) {
         if (j == end - lo) found = H[j];
     int want_low, want_high;
     if (!certify(a, n, m, found, lo, lo + W - 1, &want_low, &want_high)) {
-        file_retry(a, slot, want_low, want_high);
+        file_retry(a, slot, n + m, want_low, want_high);
         return;
     }
     a.cost[slot] = found;
     a.status[slot] = 0;
-    if (RECORD && found <= a.cap) {
-        const uint32_t *fl = flags;
-        int cached_row = -1, cached_word = -1;
-        uint32_t held = 0;
-        __threadfence();
-        walk_and_publish(a, slot, n, m, lo, [&](int r, int j) -> uint32_t {
-            // A proven band holds the path: the guard only keeps a wrong path inside the decisions.
-            if ((unsigned)j >= (unsigned)W) return 0u;
-            const int w = j >> 3;
-            if (r != cached_row || w != cached_word) {
-                held = fl[((size_t)(r - 1) * WORDS + w) * 64];
-                cached_row = r;
-                cached_word = w;
-            }
-            return (held >> (4 * (j & 7))) & 15u;
-        });
-    }
+    a.walk_low[slot] = lo;
+    a.walk_class[slot] = a.cls;
+}
+
+// The walk back for every pair a lane sweep settled, one thread a pair: a light kernel, so many waves hide the
+// walk's chain of dependent loads. A pair's decisions lie row after row, so a cache line holds several rows.
+__global__ __launch_bounds__(256) void walk_lanes(SweepArgs a) {
+    const int slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= a.slots) return;
+    if (a.status[slot] != 0) return;
+    const int cls = a.walk_class[slot];
+    if (cls > CLASS_LANE64 || a.cost[slot] > a.cap) return;
+    const int W = 16 << cls;
+    const int row_bytes = W / 2;
+    const int n = a.tape.rows[slot], m = a.tape.cols[slot];
+    const int lo = a.walk_low[slot];
+    const uint8_t *fl = a.lane_flags + a.lane_base[(size_t)cls * a.slots + slot];
+    int cached_row = -1, cached_word = -1;
+    uint32_t held = 0;
+    walk_and_publish(a, slot, n, m, lo, [&](int r, int j) -> uint32_t {
+        // A proven band holds the path: the guard only keeps a wrong path inside the decisions.
+        if ((unsigned)j >= (unsigned)W) return 0u;
+        const int w = j >> 3;
+        if (r != cached_row || w != cached_word) {
+            held = *reinterpret_cast<const uint32_t *>(fl + (size_t)(r - 1) * row_bytes + 4 * w);
+            cached_row = r;
+            cached_word = w;
+        }
+        return (held >> (4 * (j & 7))) & 15u;
+    });
 }
 
 // ------------------------------------------------------------------ one wave per pair
@@ -404,7 +431,13 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
     const bool barred_band = a.user_band != 0;
     const int user_low = a.user_low, user_high = a.user_high;
     uint8_t *flags = nullptr;
-    if (RECORD) flags = reinterpret_cast<uint8_t *>(a.flags) + a.flag_base[t];
+    if (RECORD) {
+        // The launch's arena holds this pair's decisions, W bytes a row.
+        unsigned long long at = 0;
+        if (lane == 0) at = atomicAdd(a.arena_used, ((unsigned long long)n * W + 15) & ~15ull);
+        at = __shfl(at, 0);
+        flags = a.arena + at;
+    }
 
     for (int r = 1; r <= n; ++r) {
         const uint8_t letter = letters[r - 1];
@@ -511,11 +544,13 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
     if (lane != 0) return;
     int want_low, want_high;
     if (!certify(a, n, m, found, lo, lo + W - 1, &want_low, &want_high)) {
-        file_retry(a, slot, want_low, want_high);
+        file_retry(a, slot, n + m, want_low, want_high);
         return;
     }
     a.cost[slot] = found;
     a.status[slot] = 0;
+    a.walk_low[slot] = lo;
+    a.walk_class[slot] = a.cls;
     if (RECORD && found <= a.cap) {
         const uint8_t *fl = flags;
         walk_and_publish(a, slot, n, m, lo,
@@ -560,9 +595,9 @@ int class_width(int cls) {
 bool class_is_wave(int cls) { return cls >= CLASS_WAVE_FIRST; }
 
 // Launches one class's sweep over `a.list`; `lds` the letters' bytes the wave kernel stages (n + m at most).
-void launch_sweep(int cls, const SweepArgs &a, size_t lds, hipStream_t stream) {
+void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream) {
     if (a.count <= 0) return;
-    switch (cls) {
+    switch (a.cls) {
         case CLASS_LANE16: launch_lane<16>(a, stream); break;
         case CLASS_LANE32: launch_lane<32>(a, stream); break;
         case CLASS_LANE64: launch_lane<64>(a, stream); break;
@@ -574,6 +609,12 @@ void launch_sweep(int cls, const SweepArgs &a, size_t lds, hipStream_t stream) {
         case CLASS_WAVE_FIRST + 4: launch_wave<32>(a, lds, stream); break;
         default: break;
     }
+}
+
+// Walks every pair a lane sweep settled.
+void launch_walk(const SweepArgs &a, hipStream_t stream) {
+    if (a.slots <= 0) return;
+    hipLaunchKernelGGL(walk_lanes, dim3((a.slots + 255) / 256), dim3(256), 0, stream, a);
 }
 
 }  // namespace dinara
