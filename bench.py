@@ -36,10 +36,11 @@ from dinara_align import _lib, synthetic  # noqa: E402
 MISMATCH, OPEN, EXTEND = 4, 6, 2
 VALU_PEAK_TOPS = 39.3  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one int32 op a lane a clock
 HBM_PEAK_GBS = 8000.0
-OPS_PER_CELL = 12  # the shifted global score's 7 integer ops and decision recording's 5 (SURVEY 8d convention)
-CLASS_NAMES = ["sweep_lane<16>", "sweep_lane<32>", "sweep_lane<64>", "sweep_wave<2>", "sweep_wave<4>",
-               "sweep_wave<8>", "sweep_wave<16>", "sweep_wave<32>", "sweep_wave<1>"]
-CLASS_WIDTHS = [16, 32, 64, 128, 256, 512, 1024, 2048, 64]
+OPS_PER_CELL = 12  # a gap-affine cell's 7 integer ops for its score and 5 for its decision: the counting convention
+CLASS_NAMES = ["sweep_group<1>", "sweep_group<2>", "sweep_group<4>", "sweep_group<8>", "sweep_wave<1>",
+               "sweep_wave<2>", "sweep_wave<4>", "sweep_wave<8>", "sweep_wave<16>", "sweep_wave<32>"]
+CLASS_WIDTHS = [32, 64, 128, 256, 64, 128, 256, 512, 1024, 2048]
+CLASSES = len(CLASS_NAMES)
 
 
 def host_cores() -> int:
@@ -167,10 +168,10 @@ def main() -> int:
             return 1
         device_ms.append(handle.dinara_batch_stat(batch, 0))
         walk_ms.append(handle.dinara_batch_stat(batch, 5))
-        ms = np.array([handle.dinara_batch_stat(batch, 100 + c) for c in range(9)])
+        ms = np.array([handle.dinara_batch_stat(batch, 100 + c) for c in range(CLASSES)])
         class_ms = ms if class_ms is None else class_ms + ms
-        class_launches = np.array([handle.dinara_batch_stat(batch, 200 + c) for c in range(9)])
-        class_pairs = np.array([handle.dinara_batch_stat(batch, 400 + c) for c in range(9)])
+        class_launches = np.array([handle.dinara_batch_stat(batch, 200 + c) for c in range(CLASSES)])
+        class_pairs = np.array([handle.dinara_batch_stat(batch, 400 + c) for c in range(CLASSES)])
     first_proven = handle.dinara_batch_stat(batch, 4)
     print(f"[bench] {args.steps} steps, {np.mean(walls) * 1e3:.2f} ms each", file=sys.stderr, flush=True)
 
@@ -258,7 +259,7 @@ def main() -> int:
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": "configs[1]: global gap-affine (4,6,2) scores + CIGARs, 250 bp at 5% "
+            "workload": "global gap-affine (4,6,2) scores + CIGARs, 100,000 pairs of 250 bp at 5% "
                         "(60/20/20, indels Geometric(0.7)), 2-bit packed",
             "pairs": pairs,
             "initial_band": int(os.environ.get("DINARA_INITIAL_WIDTH", "32")),
@@ -269,9 +270,9 @@ def main() -> int:
         "bit_exact_vs_cpu": exact,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels_ms_per_step": {**{CLASS_NAMES[c]: round(float(mean_class_ms[c]), 4) for c in range(9)
-                                   if mean_class_ms[c] > 0}, "walk_lanes": round(float(np.mean(walk_ms)), 4)},
-        "pairs_per_class": {CLASS_NAMES[c]: int(class_pairs[c]) for c in range(9) if class_pairs[c] > 0},
+        "kernels_ms_per_step": {**{CLASS_NAMES[c]: round(float(mean_class_ms[c]), 4) for c in range(CLASSES)
+                                   if mean_class_ms[c] > 0}, "walk_bands": round(float(np.mean(walk_ms)), 4)},
+        "pairs_per_class": {CLASS_NAMES[c]: int(class_pairs[c]) for c in range(CLASSES) if class_pairs[c] > 0},
         "pack_upload_s": round(pack_s, 3),
     }
     print(json.dumps(line))

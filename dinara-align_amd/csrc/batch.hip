@@ -127,16 +127,14 @@ void parallel_for(int64_t count, int64_t threads, const std::function<void(int64
     for (auto &t : pool) t.join();
 }
 
-int first_class_for(int need, bool lanes) {
-    if (lanes) {
-        if (need <= 16) return CLASS_LANE16;
-        if (need <= 32) return CLASS_LANE32;
-        if (need <= 64) return CLASS_LANE64;
-    } else if (need <= 64) {
-        return CLASS_WAVE_Q1;
-    }
-    for (int cls = CLASS_WAVE_FIRST; cls < CLASS_WAVE_FIRST + 5; ++cls)
-        if (class_width(cls) >= need) return cls;
+// The class a pair's first band takes: the narrowest group band holding `need` diagonals (one fewer than its width:
+// a group band starts on an even diagonal), else the narrowest wave band (sweep.hip class_for).
+int first_class_for(int need, bool groups) {
+    if (groups)
+        for (int cls = CLASS_GROUP32; cls <= CLASS_GROUP256; ++cls)
+            if (need < class_width(cls)) return cls;
+    for (int cls = CLASS_WAVE_FIRST; cls < CLASSES; ++cls)
+        if (need <= class_width(cls)) return cls;
     return CLASS_TOO_WIDE;
 }
 
@@ -167,12 +165,18 @@ struct dinara_batch {
     DeviceContext *ctx = nullptr;
     int max_letters = 0;                  // the most letters of a pair, both sequences
     int32_t max_rows = 0;
+    int group_letters = 4;                // LDS bytes a pair takes in a group launch
+    bool groups_fit = false;              // four pairs' letters fit one block's LDS, their costs the tags
+    int64_t dearest = 1;                  // the dearest single step: a substitution or a gap's first letter
+    std::vector<DeviceBuffer *> arenas;   // one per launch of a run, kept from run to run
     std::vector<int64_t> first_offset;    // where each class's first list starts in first_lists_d
     // device memory
     DeviceBuffer tape, group_base, group_ref_words, rows_d, cols_d, want_low, want_high, first_low_d,
         first_high_d, cost, status, walk_low, walk_class, cigar_scratch, cigar_scratch_base, cigars, cigar_used,
-        cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, lane_flags, lane_base, arena,
-        arena_used;
+        cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, walk_flags;
+    ~dinara_batch() {
+        for (DeviceBuffer *buffer : arenas) delete buffer;
+    }
     // the last run's results, by slot
     std::vector<int32_t> cost_h, status_h, cigar_length_h;
     std::vector<int64_t> cigar_offset_h;
@@ -273,54 +277,49 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
             }
         }
     });
-    // The first band: the lanes' start and end diagonals and one either side (lanes.mojo:1207-1215), within the
-    // user band; swept at no less than the initial width, centred.
-    b->first_low.resize(slots);
-    b->first_high.resize(slots);
-    for (auto &l : b->first_lists) l.clear();
-    b->scratch_base.resize(slots + 1);
-    int64_t scratch = 0;
-    for (int64_t s = 0; s < slots; ++s) {
-        const int32_t n = b->rows[s], m = b->cols[s];
-        const int32_t end = m - n;
-        int32_t lo = std::min(0, end) - 1, hi = std::max(0, end) + 1;
-        bool lanes = true;
-        if (b->user_band) {
-            lo = std::max(lo, b->user_low);
-            hi = std::min(hi, b->user_high);
-            lanes = b->user_low <= -n && b->user_high >= m;  // the band leaves this pair whole
-        }
-        b->first_low[s] = lo;
-        b->first_high[s] = hi;
-        const int need = std::max(hi - lo + 1, b->initial_width);
-        const int cls = first_class_for(need, lanes);
-        // A pair whose start and end diagonals lie further apart than the widest band is left unswept: its
-        // status stays unsettled and it reports DINARA_OUT_OF_MEMORY.
-        if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
-        b->scratch_base[s] = scratch;
-        if (b->kind == DINARA_BATCH_ALIGNMENTS) scratch += 2 * ((int64_t)n + m) + 2;
-    }
-    b->scratch_base[slots] = scratch;
     b->max_rows = 0;
     b->max_letters = 0;
     for (int64_t s = 0; s < slots; ++s) {
         b->max_rows = std::max(b->max_rows, b->rows[s]);
         b->max_letters = std::max<int>(b->max_letters, b->rows[s] + b->cols[s]);
     }
-    // Each lane class's decisions for every slot, rows of W/2 bytes one after another.
-    std::vector<int64_t> lane_base;
-    int64_t lane_bytes = 0;
-    if (b->kind == DINARA_BATCH_ALIGNMENTS) {
-        lane_base.resize(3 * slots);
-        for (int c = 0; c < 3; ++c) {
-            const int64_t row_bytes = class_width(c) / 2;
-            for (int64_t s = 0; s < slots; ++s) {
-                lane_base[c * slots + s] = lane_bytes;
-                lane_bytes += (int64_t)b->rows[s] * row_bytes;
-            }
-            lane_bytes = (lane_bytes + 255) & ~int64_t(255);
+    // A group pair's LDS region: both sequences and room for reads up to half the widest group band (256) past
+    // the query's end (group.hip).
+    b->group_letters = (b->max_letters + 128 + 16 + 3) & ~3;
+    // The group sweep holds costs times 16 with a tag (group.hip) and runs up to half its widest band past the
+    // corner: every value it makes stays under 2^30.
+    b->groups_fit = (size_t)4 * b->group_letters <= b->lds_limit &&
+                    ((int64_t)b->max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
+    // The first band: the lanes' start and end diagonals and one either side (lanes.mojo:1207-1215), within the
+    // user band; swept at no less than the initial width, centred.
+    b->first_low.resize(slots);
+    b->first_high.resize(slots);
+    for (auto &l : b->first_lists) l.clear();
+    b->scratch_base.resize(slots + 1);
+    int64_t scratch = 0, text = 0;
+    for (int64_t s = 0; s < slots; ++s) {
+        const int32_t n = b->rows[s], m = b->cols[s];
+        const int32_t end = m - n;
+        int32_t lo = std::min(0, end) - 1, hi = std::max(0, end) + 1;
+        if (b->user_band) {
+            lo = std::max(lo, b->user_low);
+            hi = std::min(hi, b->user_high);
+        }
+        b->first_low[s] = lo;
+        b->first_high[s] = hi;
+        const int need = std::max(hi - lo + 1, b->initial_width);
+        const int cls = first_class_for(need, !b->user_band && b->groups_fit);
+        // A pair whose start and end diagonals lie further apart than the widest band is left unswept: its
+        // status stays unsettled and it reports DINARA_OUT_OF_MEMORY.
+        if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
+        // A walk's moves, 2 bits each in 64-bit words; the CIGAR text, at most two bytes a move.
+        b->scratch_base[s] = scratch;
+        if (b->kind == DINARA_BATCH_ALIGNMENTS) {
+            scratch += 8 * (((int64_t)n + m) / 32 + 1);
+            text += 2 * ((int64_t)n + m) + 2;
         }
     }
+    b->scratch_base[slots] = scratch;
     std::vector<int32_t> first_all;
     b->first_offset.assign(CLASSES + 1, 0);
     for (int c = 0; c < CLASSES; ++c) {
@@ -345,16 +344,12 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         !b->retry_lists.reserve((size_t)2 * CLASSES * std::max<int64_t>(slots, 1) * 4) ||
         !b->retry_counts.reserve(2 * CLASSES * 4) ||
         !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4) ||
-        !b->arena_used.reserve(8))
-        return DINARA_OUT_OF_MEMORY;
-    if (b->kind == DINARA_BATCH_ALIGNMENTS &&
-        (!b->lane_flags.reserve(std::max<int64_t>(lane_bytes, 256)) ||
-         !b->lane_base.upload(ctx, lane_base.data(), lane_base.size() * 8)))
+        !b->walk_flags.reserve(std::max<int64_t>(slots, 1) * 8))
         return DINARA_OUT_OF_MEMORY;
     if (b->kind == DINARA_BATCH_ALIGNMENTS) {
         if (!b->cigar_scratch.reserve(std::max<int64_t>(scratch, 1)) ||
             !b->cigar_scratch_base.upload(ctx, b->scratch_base.data(), b->scratch_base.size() * 8) ||
-            !b->cigars.reserve(std::max<int64_t>(scratch, 1)) || !b->cigar_used.reserve(8) ||
+            !b->cigars.reserve(std::max<int64_t>(text, 1)) || !b->cigar_used.reserve(8) ||
             !b->cigar_offset.reserve(std::max<int64_t>(slots, 1) * 8) ||
             !b->cigar_length.reserve(std::max<int64_t>(slots, 1) * 4))
             return DINARA_OUT_OF_MEMORY;
@@ -387,10 +382,9 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.status = static_cast<int32_t *>(b->status.ptr);
     a.walk_low = static_cast<int32_t *>(b->walk_low.ptr);
     a.walk_class = static_cast<int32_t *>(b->walk_class.ptr);
-    a.lane_flags = static_cast<uint8_t *>(b->lane_flags.ptr);
-    a.lane_base = static_cast<const int64_t *>(b->lane_base.ptr);
-    a.arena = static_cast<uint8_t *>(b->arena.ptr);
-    a.arena_used = static_cast<unsigned long long *>(b->arena_used.ptr);
+    a.walk_flags = static_cast<uint64_t *>(b->walk_flags.ptr);
+    a.group_letters = b->group_letters;
+    a.groups_fit = b->groups_fit;
     a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
     a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
     a.cigars = static_cast<char *>(b->cigars.ptr);
@@ -401,30 +395,41 @@ SweepArgs sweep_args(dinara_batch *b) {
     return a;
 }
 
-constexpr size_t ARENA_BUDGET = size_t(16) << 30;  // decisions one wave launch may keep
+constexpr size_t ARENA_BUDGET = size_t(16) << 30;  // decisions one launch may keep
 
 // Launches class `cls` over `count` slots listed at `list` (device memory), filing retries into `retry_lists` /
-// `retry_counts`; wave launches are cut so each one's decisions fit the arena.
+// `retry_counts`. Each launch keeps its decisions in an arena of its own, a fixed stride a listed pair, until the
+// walk; a launch whose arena would pass the budget is cut in pieces. `arena_index` counts the run's arenas.
 int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t count, int32_t *retry_lists,
-                     int32_t *retry_counts) {
+                     int32_t *retry_counts, size_t &arena_index) {
     DeviceContext *ctx = b->ctx;
     SweepArgs a = sweep_args(b);
     a.cls = cls;
     a.retry_lists = retry_lists;
     a.retry_counts = retry_counts;
     const bool wave = class_is_wave(cls);
-    const size_t lds = wave ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit) : 0;
+    const size_t lds = wave ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit)
+                            : (size_t)4 * b->group_letters;
     int64_t chunk = count;
-    if (wave && a.record) {
-        const size_t per_pair = ((size_t)b->max_rows * class_width(cls) + 15) & ~size_t(15);
-        chunk = std::max<int64_t>(1, (int64_t)std::min<size_t>(count, ARENA_BUDGET / std::max<size_t>(per_pair, 1)));
-        if (!b->arena.reserve(std::max<size_t>(per_pair * chunk, 16))) return DINARA_OUT_OF_MEMORY;
-        a.arena = static_cast<uint8_t *>(b->arena.ptr);
+    size_t stride = 0;
+    if (a.record) {
+        // The batch's largest pair bounds every pair's decisions: rows for a wave band, tiles of 32
+        // anti-diagonals (16 bytes a diagonal pair) for a group's.
+        const int64_t bytes = wave ? (int64_t)b->max_rows * class_width(cls)
+                                   : ((int64_t)b->max_letters / 32 + 1) * (class_width(cls) / 2) * 16;
+        stride = ((size_t)bytes + 255) & ~size_t(255);
+        chunk = std::max<int64_t>(1, (int64_t)std::min<size_t>(count, ARENA_BUDGET / std::max<size_t>(stride, 1)));
     }
     for (int64_t start = 0; start < count; start += chunk) {
         a.list = list + start;
         a.count = (int)std::min(chunk, count - start);
-        if (wave && a.record) hipMemsetAsync(b->arena_used.ptr, 0, 8, ctx->stream);
+        if (a.record) {
+            if (arena_index == b->arenas.size()) b->arenas.push_back(new DeviceBuffer());
+            DeviceBuffer *arena = b->arenas[arena_index++];
+            if (!arena->reserve(std::max<size_t>(stride * a.count, 256))) return DINARA_OUT_OF_MEMORY;
+            a.arena = static_cast<uint8_t *>(arena->ptr);
+            a.arena_stride = (int64_t)stride;
+        }
         launch_sweep(a, lds, ctx->stream);
         if (hipGetLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
         b->launches += 1;
@@ -457,6 +462,7 @@ int64_t run(dinara_batch *b) {
     int32_t *lists = static_cast<int32_t *>(b->retry_lists.ptr);
     int32_t *count_d = static_cast<int32_t *>(b->retry_counts.ptr);
     const size_t capacity = (size_t)std::max<int64_t>(slots, 1);
+    size_t arena_index = 0;
     for (int round = 0; slots > 0 && round < 64; ++round) {
         int64_t pending = 0;
         for (int c = 0; c < CLASSES; ++c) pending += counts[c];
@@ -472,7 +478,8 @@ int64_t run(dinara_batch *b) {
                                              : in_lists + (size_t)c * capacity;
             // Each class's launches between two events of their own, for the per-kernel time.
             hipEventRecord(ctx->event(2 * c), ctx->stream);
-            if (const int64_t code = launch_class(b, c, list, counts[c], out_lists, out_counts)) return code;
+            if (const int64_t code = launch_class(b, c, list, counts[c], out_lists, out_counts, arena_index))
+                return code;
             hipEventRecord(ctx->event(2 * c + 1), ctx->stream);
         }
         hipEventRecord(ctx->sweep_stop, ctx->stream);
@@ -585,6 +592,7 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     b->user_high = (int)std::min<int64_t>(-o.band_low, clamp);
     const int64_t dearest = std::max<int64_t>(
         parsed.mismatch, std::max(parsed.opening + parsed.extension, parsed.del_opening + parsed.del_extension));
+    b->dearest = std::max<int64_t>(dearest, 1);
     b->code.assign(b->pairs, 0);
     b->reference_length.assign(b->pairs, 0);
     b->query_length.assign(b->pairs, 0);

@@ -32,13 +32,12 @@ struct Tape {
     int bits;
 };
 
-// The sweep classes: three band widths one thread per pair holds, then wave-per-pair bands of 64 * Q.
-constexpr int CLASS_LANE16 = 0;
-constexpr int CLASS_LANE32 = 1;
-constexpr int CLASS_LANE64 = 2;
-constexpr int CLASS_WAVE_FIRST = 3;  // Q = 2, 4, 8, 16, 32 for classes 3..7
-constexpr int CLASS_WAVE_Q1 = 8;     // Q = 1: a 64-diagonal band under a user band, which the lanes refuse
-constexpr int CLASSES = 9;
+// The sweep classes: four sixteen-lane-group bands (32, 64, 128, 256 diagonals, group.hip), then wave-per-pair
+// bands of 64Q (Q = 1..32, sweep.hip), which also take every pair under a user band.
+constexpr int CLASS_GROUP32 = 0;
+constexpr int CLASS_GROUP256 = 3;
+constexpr int CLASS_WAVE_FIRST = 4;  // Q = 1, 2, 4, 8, 16, 32 for classes 4..9
+constexpr int CLASSES = 10;
 constexpr int CLASS_TOO_WIDE = -1;
 
 // Everything a sweep launch reads and writes.
@@ -64,12 +63,13 @@ struct SweepArgs {
     int32_t *status;      // 0 settled, 1 filed for a wider band, 2 wider than any band class
     int32_t *walk_low;    // the settled band's lowest diagonal and its class, for the walk
     int32_t *walk_class;
-    // decisions: the lane classes' per slot, W/2 bytes a row, at lane_flags + lane_base[cls * slots + slot];
-    // the wave classes' in an arena each launch reuses, W bytes a row, from arena_used
-    uint8_t *lane_flags;
-    const int64_t *lane_base;
-    uint8_t *arena;
-    unsigned long long *arena_used;
+    // decisions: a group band's 4 bits a cell, a word for each diagonal pair every 8 anti-diagonals; a wave band's
+    // a byte a cell, W bytes a row
+    uint8_t *arena;          // this launch's decisions, `arena_stride` bytes a listed pair
+    int64_t arena_stride;
+    uint64_t *walk_flags;    // per slot: the address of its settled band's decisions
+    int group_letters;       // LDS bytes a pair takes in a group launch (four pairs a block)
+    int groups_fit;          // 1 when four pairs' letters fit one block's LDS
     // CIGARs: each slot's scratch run of 2(n+m)+2 bytes from cigar_scratch_base, compacted into `cigars` at an
     // offset taken from `cigar_used`
     char *cigar_scratch;
@@ -89,5 +89,6 @@ int class_width(int cls);
 bool class_is_wave(int cls);
 void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream);
 void launch_walk(const SweepArgs &a, hipStream_t stream);
+void launch_group(int s, const SweepArgs &a, size_t lds, hipStream_t stream);
 
 }  // namespace dinara

@@ -172,8 +172,8 @@ void dinara_batch_free(dinara_batch *batch);
 
 /* New: what the last dinara_batch_run spent, read from HIP events on the library's stream: `what` 0 the
  * whole run (ms), 1 the banded sweeps (ms), 2 the number of sweep launches, 4 the pairs the first band proved,
- * 5 the walk-back kernel (ms); for band class c (0-2 one thread a pair, 16/32/64 diagonals; 3-7 one wave a
- * pair, 128..2048; 8 one wave, 64 under a user band) 100+c its kernels' time (ms), 200+c its launches, 400+c
+ * 5 the walk-back kernel (ms); for band class c (0-3 sixteen lanes a pair, 32/64/128/256 diagonals; 4-9 one
+ * wave a pair, 64..2048) 100+c its kernels' time (ms), 200+c its launches, 400+c
  * the pairs it swept. For roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
