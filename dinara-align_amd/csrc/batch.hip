@@ -16,6 +16,7 @@
 #include "../../include/dinara.h"
 #include "common.h"
 #include "device.h"
+#include "host.h"
 
 using namespace dinara;
 
@@ -115,28 +116,24 @@ int64_t pair_code(const char *ref, int64_t n, const char *qry, int64_t m) {
     return 0;
 }
 
-void parallel_for(int64_t count, int64_t threads, const std::function<void(int64_t, int64_t)> &body) {
-    const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(threads, count));
-    if (workers <= 1) {
-        body(0, count);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (int64_t w = 0; w < workers; ++w)
-        pool.emplace_back([&, w]() { body(count * w / workers, count * (w + 1) / workers); });
-    for (auto &t : pool) t.join();
-}
-
 // The class a pair's first band takes: the narrowest group band holding `need` diagonals (one fewer than its width:
-// a group band starts on an even diagonal), else the narrowest wave band (sweep.hip class_for).
-int first_class_for(int need, bool groups) {
+// a group band starts on an even diagonal), else the narrowest block band, else the narrowest wave band (as the
+// device's class_for).
+int first_class_for(int need, bool groups, int block_mask, bool user_band) {
     if (groups)
         for (int cls = CLASS_GROUP32; cls <= CLASS_GROUP256; ++cls)
             if (need < class_width(cls)) return cls;
-    for (int cls = CLASS_WAVE_FIRST; cls < CLASSES; ++cls)
+    if (!user_band)
+        for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES; ++cls)
+            if (((block_mask >> (cls - CLASS_BLOCK_FIRST)) & 1) && need < class_width(cls)) return cls;
+    for (int cls = CLASS_WAVE_FIRST; cls <= CLASS_WAVE_LAST; ++cls)
         if (need <= class_width(cls)) return cls;
     return CLASS_TOO_WIDE;
 }
+
+// LDS bytes one block of class `cls` stages: four pairs' letters for a group, one pair's and the reads past it
+// for a block.
+size_t block_letters(int max_letters, int cls) { return ((size_t)max_letters + class_width(cls) / 2 + 16 + 3) & ~size_t(3); }
 
 }  // namespace
 
@@ -167,6 +164,7 @@ struct dinara_batch {
     int32_t max_rows = 0;
     int group_letters = 4;                // LDS bytes a pair takes in a group launch
     bool groups_fit = false;              // four pairs' letters fit one block's LDS, their costs the tags
+    int block_mask = 0;                   // the block classes whose letters and tags fit
     int64_t dearest = 1;                  // the dearest single step: a substitution or a gap's first letter
     std::vector<DeviceBuffer *> arenas;   // one per launch of a run, kept from run to run
     std::vector<int64_t> first_offset;    // where each class's first list starts in first_lists_d
@@ -290,6 +288,12 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     // corner: every value it makes stays under 2^30.
     b->groups_fit = (size_t)4 * b->group_letters <= b->lds_limit &&
                     ((int64_t)b->max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
+    b->block_mask = 0;
+    for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES; ++cls)
+        if (block_letters(b->max_letters, cls) <= b->lds_limit &&
+            ((int64_t)b->max_letters + class_width(cls) + 8) * b->dearest < REAL_LIMIT &&
+            !(cls == CLASSES - 1 && b->kind == DINARA_BATCH_ALIGNMENTS))  // the widest keeps no decisions
+            b->block_mask |= 1 << (cls - CLASS_BLOCK_FIRST);
     // The first band: the lanes' start and end diagonals and one either side (lanes.mojo:1207-1215), within the
     // user band; swept at no less than the initial width, centred.
     b->first_low.resize(slots);
@@ -308,7 +312,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         b->first_low[s] = lo;
         b->first_high[s] = hi;
         const int need = std::max(hi - lo + 1, b->initial_width);
-        const int cls = first_class_for(need, !b->user_band && b->groups_fit);
+        const int cls = first_class_for(need, !b->user_band && b->groups_fit, b->block_mask, b->user_band);
         // A pair whose start and end diagonals lie further apart than the widest band is left unswept: its
         // status stays unsettled and it reports DINARA_OUT_OF_MEMORY.
         if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
@@ -385,6 +389,7 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.walk_flags = static_cast<uint64_t *>(b->walk_flags.ptr);
     a.group_letters = b->group_letters;
     a.groups_fit = b->groups_fit;
+    a.block_mask = b->block_mask;
     a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
     a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
     a.cigars = static_cast<char *>(b->cigars.ptr);
@@ -395,7 +400,6 @@ SweepArgs sweep_args(dinara_batch *b) {
     return a;
 }
 
-constexpr size_t ARENA_BUDGET = size_t(16) << 30;  // decisions one launch may keep
 
 // Launches class `cls` over `count` slots listed at `list` (device memory), filing retries into `retry_lists` /
 // `retry_counts`. Each launch keeps its decisions in an arena of its own, a fixed stride a listed pair, until the
@@ -408,8 +412,9 @@ int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t coun
     a.retry_lists = retry_lists;
     a.retry_counts = retry_counts;
     const bool wave = class_is_wave(cls);
-    const size_t lds = wave ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit)
-                            : (size_t)4 * b->group_letters;
+    const size_t lds = wave                     ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit)
+                       : band_is_block(cls) ? block_letters(b->max_letters, cls)
+                                            : (size_t)4 * b->group_letters;
     int64_t chunk = count;
     size_t stride = 0;
     if (a.record) {

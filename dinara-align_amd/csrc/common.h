@@ -32,13 +32,34 @@ struct Tape {
     int bits;
 };
 
-// The sweep classes: four sixteen-lane-group bands (32, 64, 128, 256 diagonals, group.hip), then wave-per-pair
-// bands of 64Q (Q = 1..32, sweep.hip), which also take every pair under a user band.
+// The sweep classes: four sixteen-lane-group bands (32, 64, 128, 256 diagonals, group.hip); wave-per-pair bands of
+// 64Q (Q = 1..32, sweep.hip), which also take every pair under a user band; and block bands, one pair a block of
+// waves (group.hip): 512 (S = 4, one wave), 1024 to 8192 (S = 8, 1 to 8 waves), 16384 (S = 16, 8 waves).
 constexpr int CLASS_GROUP32 = 0;
 constexpr int CLASS_GROUP256 = 3;
 constexpr int CLASS_WAVE_FIRST = 4;  // Q = 1, 2, 4, 8, 16, 32 for classes 4..9
-constexpr int CLASSES = 10;
+constexpr int CLASS_WAVE_LAST = 9;
+constexpr int CLASS_BLOCK_FIRST = 10;
+constexpr int CLASSES = 16;
 constexpr int CLASS_TOO_WIDE = -1;
+
+// The diagonals a class's band holds.
+__host__ __device__ constexpr int band_width(int cls) {
+    return cls <= CLASS_GROUP256    ? 32 << cls
+           : cls <= CLASS_WAVE_LAST ? 64 << (cls - CLASS_WAVE_FIRST)
+           : cls == CLASS_BLOCK_FIRST ? 512
+                                      : 1024 << (cls - CLASS_BLOCK_FIRST - 1);
+}
+__host__ __device__ constexpr bool band_is_wave(int cls) { return cls >= CLASS_WAVE_FIRST && cls <= CLASS_WAVE_LAST; }
+__host__ __device__ constexpr bool band_is_block(int cls) { return cls >= CLASS_BLOCK_FIRST; }
+// A block class's diagonal pairs a lane and waves a pair: 512 = 4 x 1 wave, 1024 to 8192 = 8 x 1 to 8 waves,
+// 16384 = 16 x 8 waves (a block of at most 8 waves keeps 256 registers a lane).
+__host__ __device__ constexpr int block_s(int cls) {
+    return cls == CLASS_BLOCK_FIRST ? 4 : (cls == CLASSES - 1 ? 16 : 8);
+}
+__host__ __device__ constexpr int block_waves(int cls) {
+    return cls == CLASS_BLOCK_FIRST ? 1 : (cls == CLASSES - 1 ? 8 : 1 << (cls - CLASS_BLOCK_FIRST - 1));
+}
 
 // Everything a sweep launch reads and writes.
 struct SweepArgs {
@@ -70,6 +91,7 @@ struct SweepArgs {
     uint64_t *walk_flags;    // per slot: the address of its settled band's decisions
     int group_letters;       // LDS bytes a pair takes in a group launch (four pairs a block)
     int groups_fit;          // 1 when four pairs' letters fit one block's LDS
+    int block_mask;          // bit c - CLASS_BLOCK_FIRST: block class c's letters fit its LDS and its costs its tags
     // CIGARs: each slot's scratch run of 2(n+m)+2 bytes from cigar_scratch_base, compacted into `cigars` at an
     // offset taken from `cigar_used`
     char *cigar_scratch;
@@ -90,5 +112,6 @@ bool class_is_wave(int cls);
 void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream);
 void launch_walk(const SweepArgs &a, hipStream_t stream);
 void launch_group(int s, const SweepArgs &a, size_t lds, hipStream_t stream);
+void launch_block(int s, int waves, const SweepArgs &a, size_t lds, hipStream_t stream);
 
 }  // namespace dinara

@@ -39,13 +39,17 @@ namespace dinara {
 // Tagged unreachable: above every real tagged value (the host keeps costs times 16 under 2^30) with room to add.
 constexpr int TAG_INF = 1 << 30;
 
-// Keeps the old value where the row has no lane below (row_shr:1): lane 0's register never changes.
+// The neighbour one lane below: within a 16-lane row (row_shr:1), or across the wave (wave_shr:1). The first lane
+// has no source and keeps the old value: its register never changes.
+template <bool WAVE>
 __device__ __forceinline__ int shift_from_lower(int old, int v) {
-    return __builtin_amdgcn_update_dpp(old, v, 0x111, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(old, v, WAVE ? 0x138 : 0x111, 0xF, 0xF, false);
 }
-// Keeps the old value where the row has no lane above (row_shl:1): lane 15's register never changes.
+// The neighbour one lane above: within the row (row_shl:1), or across the wave (wave_shl:1); the last lane keeps
+// the old value.
+template <bool WAVE>
 __device__ __forceinline__ int shift_from_upper(int old, int v) {
-    return __builtin_amdgcn_update_dpp(old, v, 0x101, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(old, v, WAVE ? 0x130 : 0x101, 0xF, 0xF, false);
 }
 
 // The tagged constants a sweep adds.
@@ -138,13 +142,13 @@ struct GroupState {
 };
 
 // Even anti-diagonal: even slot s's cell takes D from odd slot s (one diagonal up) and I from odd slot s - 1
-// (the lane below's last for slot 0). The careful form captures the corner's tagged H when it is this cell.
+// (the lane below's last for slot 0, in from_lower_i, which the caller has shifted in). The careful form captures
+// the corner's tagged H when it is this cell.
 template <int S, bool RECORD, bool SH, bool CAREFUL>
 __device__ __forceinline__ void even_half(GroupState<S> &st, const GroupCosts &g, int t, int base_k,
                                           const uint8_t *ref, const uint8_t *qry, int last_ref, int last_qry,
                                           const uint8_t *rp, const uint8_t *qp, int first_q, int q_end, int tend,
                                           int &corner) {
-    st.from_lower_i = shift_from_lower(st.from_lower_i, st.Io[S - 1]);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int ix = s > 0 ? st.Io[s - 1] : st.from_lower_i;
@@ -164,14 +168,13 @@ __device__ __forceinline__ void even_half(GroupState<S> &st, const GroupCosts &g
     }
 }
 
-// Odd anti-diagonal: odd slot s's cell takes D from even slot s + 1 (the lane above's first for the last slot)
-// and I from even slot s.
+// Odd anti-diagonal: odd slot s's cell takes D from even slot s + 1 (the lane above's first for the last slot, in
+// from_upper_d, which the caller has shifted in) and I from even slot s.
 template <int S, bool RECORD, bool SH, bool CAREFUL>
 __device__ __forceinline__ void odd_half(GroupState<S> &st, const GroupCosts &g, int t, int base_k,
                                          const uint8_t *ref, const uint8_t *qry, int last_ref, int last_qry,
                                          const uint8_t *rp, const uint8_t *qp, int first_q, int q_end, int tend,
                                          int &corner) {
-    st.from_upper_d = shift_from_upper(st.from_upper_d, st.De[0]);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const int dx = s + 1 < S ? st.De[s + 1] : st.from_upper_d;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     if (RECORD) tiles = reinterpret_cast<uint4 *>(a.arena + (size_t)index * a.arena_stride);
     const int last_tile = tend >> 5;
 
-    auto store_tile = [&](int tile) {
+    auto store_tile = [&](int tile) __attribute__((always_inline)) {
         if (RECORD && live && tile <= last_tile) {
 #pragma unroll
             for (int s = 0; s < S; ++s)
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
         }
     };
     // After odd anti-diagonal t = 8b + 7 block b is whole: kept, or the tile stored with it.
-    auto finish_block = [&](int t) {
+    auto finish_block = [&](int t) __attribute__((always_inline)) {
         const int block = t >> 3, place = block & 3;
         if (place == 3) {
             store_tile(block >> 2);
@@ -280,9 +283,11 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
             }
         }
     };
-    auto careful_pair = [&](int t) {
+    auto careful_pair = [&](int t) __attribute__((always_inline)) {
+        st.from_lower_i = shift_from_lower<false>(st.from_lower_i, st.Io[S - 1]);
         even_half<S, RECORD, SH, true>(st, g, t, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
                                        q_end, tend, corner);
+        st.from_upper_d = shift_from_upper<false>(st.from_upper_d, st.De[0]);
         odd_half<S, RECORD, SH, true>(st, g, t + 1, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
                                       q_end, tend, corner);
         if (((t + 1) & 7) == 7) finish_block(t + 1);
@@ -295,12 +300,14 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     // four blocks at a time, their blocks' places known.
     if (t <= tmax) {
         int r0 = (t - base_k) >> 1, c0 = r0 + base_k;
-        auto steady_block = [&]() {
+        auto steady_block = [&]() __attribute__((always_inline)) {
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
                 const uint8_t *rp = ref + r0 - 1, *qp = qry + c0 - 1;
+                st.from_lower_i = shift_from_lower<false>(st.from_lower_i, st.Io[S - 1]);
                 even_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
                                                 q_end, tend, corner);
+                st.from_upper_d = shift_from_upper<false>(st.from_upper_d, st.De[0]);
                 odd_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
                                                q_end, tend, corner);
                 ++r0;
@@ -356,6 +363,191 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     a.walk_low[slot] = k0;
     a.walk_class[slot] = a.cls;
     a.walk_flags[slot] = reinterpret_cast<uint64_t>(tiles);
+}
+
+// The same sweep over a wide band, one pair a block of G <= 8 waves: the band's W = 128SG diagonals dealt out as above
+// to 64G lanes, lane j of the block holding diagonal pairs k0 + 2(Sj + s). Inside a wave the neighbours come over
+// with wave_shr:1 / wave_shl:1; across a wave boundary through LDS, each half-step behind a barrier (the edge lane
+// writes its value, the neighbour wave's edge lane reads it after the barrier; the next write of that slot comes
+// after the other half-step's barrier, which the reader passes only once it has read).
+template <int S, bool RECORD, bool SH>
+__global__ __launch_bounds__(512) void sweep_block(SweepArgs a) {
+    extern __shared__ uint8_t letters[];
+    __shared__ int edge_i[8], edge_d[8], found_at;
+    const int waves = (int)(blockDim.x >> 6), wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int W = 128 * S * waves;
+    const int index = blockIdx.x;
+    const int slot = a.list[index];
+    const int n = a.tape.rows[slot], m = a.tape.cols[slot];
+    const int need_low = a.want_low[slot], need_high = a.want_high[slot];
+    int k0 = need_low - (W - (need_high - need_low + 1)) / 2;
+    k0 -= k0 & 1;
+    const int end = m - n;
+    const GroupCosts g = group_costs<RECORD, SH>(a.costs);
+
+    uint8_t *ref = letters;
+    uint8_t *qry = ref + n;
+    {
+        const int group = slot >> 6, column = slot & 63;
+        const uint32_t *ref_words = a.tape.words + a.tape.group_base[group] + column;
+        const uint32_t *qry_words = ref_words + (size_t)a.tape.group_ref_words[group] * 64;
+        const int bits = a.tape.bits, per = 32 / bits;
+        const uint32_t mask = (1u << bits) - 1;
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            ref[i] = (uint8_t)((ref_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+        for (int i = threadIdx.x; i < m; i += blockDim.x)
+            qry[i] = (uint8_t)((qry_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+    }
+    __syncthreads();
+
+    const int tend = n + m, tmax = tend, tmin = tend;
+    int careful_until = (k0 + W + 1) > (2 - k0) ? (k0 + W + 1) : (2 - k0);
+    careful_until = (careful_until + 7) & ~7;
+    const int last_ref = n > 0 ? n - 1 : 0, last_qry = m > 0 ? m - 1 : 0;
+    const int gl = (int)threadIdx.x;       // the lane within the pair's band
+    const int base_k = k0 + 2 * S * gl;
+    const int q_end = (end - k0) >> 1;
+    const int first_q = S * gl;
+
+    GroupState<S> st;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        st.He[s] = st.De[s] = st.Ie[s] = st.Ho[s] = st.Do[s] = st.Io[s] = TAG_INF;
+        st.acc[s] = 0;
+        st.done[s][0] = st.done[s][1] = st.done[s][2] = 0;
+    }
+    st.from_lower_i = TAG_INF;
+    st.from_upper_d = TAG_INF;
+    int corner = TAG_INF;
+    uint4 *tiles = nullptr;
+    if (RECORD) tiles = reinterpret_cast<uint4 *>(a.arena + (size_t)index * a.arena_stride);
+    const int last_tile = tend >> 5;
+
+    auto store_tile = [&](int tile) __attribute__((always_inline)) {
+        if (RECORD && tile <= last_tile) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                tiles[(size_t)tile * (W / 2) + first_q + s] =
+                    make_uint4(st.done[s][0], st.done[s][1], st.done[s][2], st.acc[s]);
+        }
+    };
+    auto finish_block = [&](int t) __attribute__((always_inline)) {
+        const int block = t >> 3, place = block & 3;
+        if (place == 3) {
+            store_tile(block >> 2);
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                st.done[s][0] = place == 0 ? st.acc[s] : st.done[s][0];
+                st.done[s][1] = place == 1 ? st.acc[s] : st.done[s][1];
+                st.done[s][2] = place == 2 ? st.acc[s] : st.done[s][2];
+            }
+        }
+    };
+    // The neighbours for the next half-steps: in-wave by DPP, across a wave boundary through LDS.
+    auto take_lower = [&]() __attribute__((always_inline)) {
+        if (waves > 1) {
+            if (lane == 63) edge_i[wave] = st.Io[S - 1];
+            __syncthreads();
+        }
+        st.from_lower_i = shift_from_lower<true>(st.from_lower_i, st.Io[S - 1]);
+        if (waves > 1 && lane == 0 && wave > 0) st.from_lower_i = edge_i[wave - 1];
+    };
+    auto take_upper = [&]() __attribute__((always_inline)) {
+        if (waves > 1) {
+            if (lane == 0) edge_d[wave] = st.De[0];
+            __syncthreads();
+        }
+        st.from_upper_d = shift_from_upper<true>(st.from_upper_d, st.De[0]);
+        if (waves > 1 && lane == 63 && wave < waves - 1) st.from_upper_d = edge_d[wave + 1];
+    };
+    auto careful_pair = [&](int t) __attribute__((always_inline)) {
+        take_lower();
+        even_half<S, RECORD, SH, true>(st, g, t, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
+                                       q_end, tend, corner);
+        take_upper();
+        odd_half<S, RECORD, SH, true>(st, g, t + 1, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
+                                      q_end, tend, corner);
+        if (((t + 1) & 7) == 7) finish_block(t + 1);
+    };
+    int t = 0;
+    for (; t <= tmax && t < careful_until; t += 2) careful_pair(t);
+    if (t <= tmax) {
+        int r0 = (t - base_k) >> 1, c0 = r0 + base_k;
+        auto steady_block = [&]() __attribute__((always_inline)) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                const uint8_t *rp = ref + r0 - 1, *qp = qry + c0 - 1;
+                take_lower();
+                even_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
+                                                q_end, tend, corner);
+                take_upper();
+                odd_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
+                                               q_end, tend, corner);
+                ++r0;
+                ++c0;
+            }
+        };
+        // Blocks of 8 anti-diagonals (the barriers already order the half-steps; a longer unroll only keeps more
+        // registers live).
+        for (; t + 7 < tmin; t += 8) {
+            steady_block();
+            finish_block(t + 7);
+        }
+    }
+    for (; t <= tmax; t += 2) careful_pair(t);
+    const int last = t - 1, filled = (last & 7) + 1, place = (last >> 3) & 3;
+    if (RECORD && last_tile == (last >> 5) && !(filled == 8 && place == 3)) {
+        if (filled != 8) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) st.acc[s] >>= 4 * (8 - filled);
+            finish_block(last | 7);
+        }
+        if (place != 3) store_tile(last_tile);
+    }
+
+    // The corner's lane hands its cost to lane 0, which certifies the band.
+    if (gl == q_end / S) found_at = corner;
+    __syncthreads();
+    if (gl != 0) return;
+    const int tagged = found_at;
+    const int found = tagged >= TAG_INF / 2 ? INF : (tagged >> 4) + (SH ? a.costs.del_ext * tend : 0);
+    int want_low, want_high;
+    if (!certify(a, n, m, found, k0, k0 + W - 1, &want_low, &want_high)) {
+        file_retry(a, slot, n + m, want_low, want_high);
+        return;
+    }
+    a.cost[slot] = found;
+    a.status[slot] = 0;
+    a.walk_low[slot] = k0;
+    a.walk_class[slot] = a.cls;
+    a.walk_flags[slot] = reinterpret_cast<uint64_t>(tiles);
+}
+
+// Launches a block class: S diagonal pairs a lane, `waves` waves a pair; `lds` one pair's letters and reads past.
+void launch_block(int s, int waves, const SweepArgs &a, size_t lds, hipStream_t stream) {
+    const dim3 grid(a.count), block(64 * waves);
+    const bool shifted = a.costs.del_ext == a.costs.ins_ext;
+#define DINARA_BLOCK(SV)                                                                                       \
+    if (a.record) {                                                                                          \
+        if (shifted) hipLaunchKernelGGL((sweep_block<SV, true, true>), grid, block, lds, stream, a);           \
+        else hipLaunchKernelGGL((sweep_block<SV, true, false>), grid, block, lds, stream, a);                 \
+    } else {                                                                                                 \
+        if (shifted) hipLaunchKernelGGL((sweep_block<SV, false, true>), grid, block, lds, stream, a);          \
+        else hipLaunchKernelGGL((sweep_block<SV, false, false>), grid, block, lds, stream, a);                \
+    }
+    switch (s) {
+        case 4: DINARA_BLOCK(4) break;
+        case 8: DINARA_BLOCK(8) break;
+        case 16:  // distances only: a band this wide keeps no decisions (its record would spill registers)
+            if (!a.record) {
+                if (shifted) hipLaunchKernelGGL((sweep_block<16, false, true>), grid, block, lds, stream, a);
+                else hipLaunchKernelGGL((sweep_block<16, false, false>), grid, block, lds, stream, a);
+            }
+            break;
+        default: break;
+    }
+#undef DINARA_BLOCK
 }
 
 template <int S>

@@ -190,10 +190,10 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
         const int n = a.tape.rows[slot], m = a.tape.cols[slot];
         const int lo = a.walk_low[slot];
         const uint8_t *base = reinterpret_cast<const uint8_t *>(a.walk_flags[slot]);
-        if (cls <= CLASS_GROUP256) {
+        if (!band_is_wave(cls)) {
             // A group band: tiles of 32 anti-diagonals, 16 bytes a diagonal pair q = (k - k0) / 2 at
             // (t / 32) * (W / 2) + q, word (t / 8) % 4, nibble t % 8 (group.hip).
-            const int half = 16 << cls;
+            const int half = band_width(cls) / 2;
             const uint4 *tiles = reinterpret_cast<const uint4 *>(base);
             int cached = -1, ahead_at = -1;
             uint4 held = make_uint4(0, 0, 0, 0), ahead = held;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
                 return (word >> (4 * (t & 7))) & 15u;
             });
         } else {
-            const int W = 64 << (cls - CLASS_WAVE_FIRST);
+            const int W = band_width(cls);
             moves = walk_rows(a, slot, n, m, lo, [&](int r, int j) -> uint32_t {
                 return (unsigned)j < (unsigned)W ? base[(size_t)(r - 1) * W + j] : 0u;
             });
@@ -233,9 +233,9 @@ static void launch_wave(const SweepArgs &a, size_t lds, hipStream_t stream) {
     else hipLaunchKernelGGL((sweep_wave<Q, false>), grid, block, lds, stream, a);
 }
 
-int class_width(int cls) { return cls <= CLASS_GROUP256 ? 32 << cls : 64 << (cls - CLASS_WAVE_FIRST); }
+int class_width(int cls) { return band_width(cls); }
 
-bool class_is_wave(int cls) { return cls >= CLASS_WAVE_FIRST; }
+bool class_is_wave(int cls) { return band_is_wave(cls); }
 
 // Launches one class's sweep over `a.list`; `lds` the letters' bytes a block stages.
 void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream) {
@@ -251,6 +251,12 @@ void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream) {
         case CLASS_WAVE_FIRST + 3: launch_wave<8>(a, lds, stream); break;
         case CLASS_WAVE_FIRST + 4: launch_wave<16>(a, lds, stream); break;
         case CLASS_WAVE_FIRST + 5: launch_wave<32>(a, lds, stream); break;
+        case CLASS_BLOCK_FIRST + 0:
+        case CLASS_BLOCK_FIRST + 1:
+        case CLASS_BLOCK_FIRST + 2:
+        case CLASS_BLOCK_FIRST + 3:
+        case CLASS_BLOCK_FIRST + 4:
+        case CLASS_BLOCK_FIRST + 5: launch_block(block_s(a.cls), block_waves(a.cls), a, lds, stream); break;
         default: break;
     }
 }

@@ -105,6 +105,18 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_free.restype = None
     handle.dinara_batch_stat.argtypes = [vp, i64]
     handle.dinara_batch_stat.restype = ctypes.c_double
+    handle.dinara_scoring_batch_new.argtypes = [i64, i64, *batch_args, vp, vp, i64, i64, pi64]
+    handle.dinara_scoring_batch_new.restype = vp
+    handle.dinara_scoring_batch_run.argtypes = [vp]
+    handle.dinara_scoring_batch_run.restype = i64
+    handle.dinara_scoring_batch_scores.argtypes = [vp, pi64]
+    handle.dinara_scoring_batch_scores.restype = i64
+    handle.dinara_scoring_batch_alignments.argtypes = [vp, vp, vp, vp, pi64]
+    handle.dinara_scoring_batch_alignments.restype = i64
+    handle.dinara_scoring_batch_free.argtypes = [vp]
+    handle.dinara_scoring_batch_free.restype = None
+    handle.dinara_scoring_batch_stat.argtypes = [vp, i64]
+    handle.dinara_scoring_batch_stat.restype = ctypes.c_double
     _LIB = handle
     return handle
 

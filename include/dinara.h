@@ -136,7 +136,9 @@ typedef struct {
 } dinara_scoring;
 
 /* New: every pair's best score under `scoring` (global, or DINARA_LOCAL with match_score zero), mirroring
- * the reference's Mojo api.scores (reference api.mojo:843-892). Per-pair codes in results as for distances. */
+ * the reference's Mojo api.scores (reference api.mojo:843-892). Scores may be below zero, so a pair that cannot
+ * be scored fails the whole call with its code (the first such pair's in the batch's order, as the reference's
+ * batch raises), and nothing is written. */
 int64_t dinara_gpu_scores(int64_t gpu, int64_t pairs, const char *const *references,
                           const int64_t *reference_lengths, const char *const *queries, const int64_t *query_lengths,
                           const dinara_scoring *scoring, const dinara_mode *mode, int64_t threads, int64_t *scores);
@@ -149,6 +151,31 @@ int64_t dinara_gpu_scoring_alignments(int64_t gpu, int64_t pairs, const char *co
                                       const int64_t *query_lengths, const dinara_scoring *scoring,
                                       const dinara_mode *mode, int64_t eqx, int64_t threads,
                                       dinara_alignment *alignments, int64_t *statuses);
+
+/*
+ * New: a Scoring batch kept resident on the device, as dinara_batch below is for costs: dinara_scoring_batch_new
+ * validates, packs and uploads (kind DINARA_BATCH_DISTANCES for scores, DINARA_BATCH_ALIGNMENTS for alignments);
+ * dinara_scoring_batch_run runs every pass on the device; _scores / _alignments copy the last run's results out in
+ * the caller's order (a local alignment's spans are found in `references`/`queries`, which must be the batch's own
+ * letters). A pair whose scores could pass 32 bits (reference scoring.mojo:216-238) reports DINARA_INVALID_COSTS;
+ * one whose reference passes what a block's LDS holds, DINARA_OUT_OF_MEMORY. dinara_scoring_batch_stat: `what` 0
+ * the run (ms), 2 the launches, 10-13 the global, local-end, reach and trace passes (ms). No counterpart in the
+ * reference.
+ */
+typedef struct dinara_scoring_batch dinara_scoring_batch;
+
+dinara_scoring_batch *dinara_scoring_batch_new(int64_t gpu, int64_t kind, int64_t pairs,
+                                               const char *const *references, const int64_t *reference_lengths,
+                                               const char *const *queries, const int64_t *query_lengths,
+                                               const dinara_scoring *scoring, const dinara_mode *mode, int64_t eqx,
+                                               int64_t threads, int64_t *status);
+int64_t dinara_scoring_batch_run(dinara_scoring_batch *batch);
+int64_t dinara_scoring_batch_scores(dinara_scoring_batch *batch, int64_t *scores);
+int64_t dinara_scoring_batch_alignments(dinara_scoring_batch *batch, const char *const *references,
+                                        const char *const *queries, dinara_alignment *alignments,
+                                        int64_t *statuses);
+void dinara_scoring_batch_free(dinara_scoring_batch *batch);
+double dinara_scoring_batch_stat(dinara_scoring_batch *batch, int64_t what);
 
 /*
  * New: a batch kept resident on the device, for callers that run the same pairs more than once (and for

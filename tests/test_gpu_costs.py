@@ -6,6 +6,7 @@ pair) and its known answers, the benchmark configurations' shapes, and the edges
 letters outside ACGT, bands, caps, both tie rules, `M` CIGARs, and bands wide enough for the wave kernel.
 """
 
+import ctypes
 import json
 import os
 import random
@@ -168,7 +169,7 @@ def test_ragged_lengths_and_empties():
     check_alignments(refs, qrys, da.Costs.edit(), ties="right")
 
 
-def test_wide_bands_take_the_wave_kernel():
+def test_wide_bands():
     rng = random.Random(29)
     refs, qrys = random_pairs(40, rng, lengths=(300, 900), rates=(0.3, 0.5, 0.8))
     # Length differences the lanes' widest band cannot hold.
@@ -176,6 +177,52 @@ def test_wide_bands_take_the_wave_kernel():
     qrys += ["ACGT" * 10, "C" * 150]
     check_alignments(refs, qrys, da.Costs.affine(4, 6, 2))
     check_alignments(refs, qrys, da.Costs.edit(), ties="right")
+    check_distances(refs, qrys, da.Costs.edit())
+
+
+def class_pairs(refs, qrys, costs, kind):
+    """Runs a resident batch through the C ABI and returns how many pairs each band class swept."""
+    from dinara_align import _lib
+    handle = _lib.lib()
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    c = costs._c()
+    status = ctypes.c_int64(0)
+    batch = handle.dinara_batch_new(0, kind, len(refs), *R.args(), *Q.args(), ctypes.byref(c), None, None, 4,
+                                    ctypes.byref(status))
+    assert batch, status.value
+    try:
+        assert handle.dinara_batch_run(batch) == 0
+        return [int(handle.dinara_batch_stat(batch, 400 + cls)) for cls in range(16)]
+    finally:
+        handle.dinara_batch_free(batch)
+
+
+def test_block_bands_one_and_several_waves():
+    # Long, noisy pairs: bands of 512 to 8192 diagonals, one pair a block of one to eight waves.
+    rng = random.Random(37)
+    refs, qrys = [], []
+    for n, rate in [(1200, 0.25), (2500, 0.3), (3000, 0.45), (1800, 0.6), (4000, 0.5), (900, 0.8)]:
+        r = "".join(rng.choice("ACGT") for _ in range(n))
+        refs.append(r)
+        qrys.append(mutated(r, rate, rng))
+    # Length differences that put the first band past one wave's.
+    refs += ["ACGT" * 700, "G" * 2500, "ACG" * 900]
+    qrys += ["ACGT" * 40, "GT" * 200, "TTAC" * 300]
+    for costs in (da.Costs.affine(4, 6, 2), da.Costs.edit()):
+        swept = class_pairs(refs, qrys, costs, 1)
+        assert sum(swept[11:16]) > 0 and sum(swept[12:16]) > 0, swept  # several-wave blocks ran
+        check_alignments(refs, qrys, costs)
+        check_distances(refs, qrys, costs)
+
+
+def test_widest_band_distances():
+    # First bands past 8192 diagonals: the 16384-diagonal block, distances only.
+    rng = random.Random(41)
+    refs = ["".join(rng.choice("ACGT") for _ in range(n)) for n in (11000, 9500)]
+    qrys = [refs[0][:1500], mutated(refs[1][2000:3200], 0.1, rng)]
+    swept = class_pairs(refs, qrys, da.Costs.affine(4, 6, 2), 0)
+    assert swept[15] > 0, swept
+    check_distances(refs, qrys, da.Costs.affine(4, 6, 2))
     check_distances(refs, qrys, da.Costs.edit())
 
 
