@@ -17,6 +17,7 @@
 #include "common.h"
 #include "device.h"
 #include "host.h"
+#include "learned.h"
 
 using namespace dinara;
 
@@ -158,6 +159,7 @@ struct dinara_batch {
     std::vector<int32_t> first_lists[CLASSES];
     std::vector<int64_t> scratch_base;
     int bits = 2;
+    bool reversed = false;                // the tape holds both sequences back to front (Ties.LEFT)
     size_t lds_limit = 0;
     DeviceContext *ctx = nullptr;
     int max_letters = 0;                  // the most letters of a pair, both sequences
@@ -249,16 +251,42 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     std::vector<uint32_t> tape(std::max<int64_t>(words, 1), 0);
     // Ties.LEFT is the rule over both sequences reversed, so their tape is reversed (lanes.mojo:1523-1525).
     const bool reversed = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
+    b->reversed = reversed;
     parallel_for(groups, b->threads, [&](int64_t lo, int64_t hi) {
         for (int64_t g = lo; g < hi; ++g) {
             for (int64_t s = g * 64; s < std::min(slots, (g + 1) * 64); ++s) {
                 const int32_t p = order[s];
+                // Two-bit codes: A, C, G and T keep 0, 1, 2, 3 (so a code names the same base in every pair, as
+                // the k-mer features read them), any other byte the least code its pair leaves free. Only
+                // equality counts to the sweeps.
                 int16_t codes[256];
                 for (int i = 0; i < 256; ++i) codes[i] = -1;
-                int next = 0;
+                if (b->bits == 2) {
+                    bool used[4] = {false, false, false, false};
+                    auto fixed = [](unsigned char byte) { return byte == 'A' ? 0 : byte == 'C' ? 1 : byte == 'G' ? 2 : byte == 'T' ? 3 : -1; };
+                    auto note = [&](const char *text, int64_t len, bool fixed_pass) {
+                        const unsigned char *u = reinterpret_cast<const unsigned char *>(text);
+                        for (int64_t i = 0; i < len; ++i) {
+                            if (codes[u[i]] >= 0) continue;
+                            const int f = fixed(u[i]);
+                            if (fixed_pass && f >= 0) {
+                                codes[u[i]] = (int16_t)f;
+                                used[f] = true;
+                            } else if (!fixed_pass) {
+                                int free_code = 0;
+                                while (used[free_code]) ++free_code;
+                                codes[u[i]] = (int16_t)free_code;
+                                used[free_code] = true;
+                            }
+                        }
+                    };
+                    note(refs[p], b->reference_length[p], true);
+                    note(qrys[p], b->query_length[p], true);
+                    note(refs[p], b->reference_length[p], false);
+                    note(qrys[p], b->query_length[p], false);
+                }
                 auto code_of = [&](unsigned char byte) -> uint32_t {
                     if (b->bits == 8) return byte;
-                    if (codes[byte] < 0) codes[byte] = (int16_t)next++;
                     return (uint32_t)codes[byte];
                 };
                 uint32_t *ref_out = tape.data() + gbase[g] + (s & 63);
@@ -331,6 +359,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         first_all.insert(first_all.end(), b->first_lists[c].begin(), b->first_lists[c].end());
     }
     b->first_offset[CLASSES] = (int64_t)first_all.size();
+    if (first_all.empty()) first_all.push_back(0);  // every pair too wide: the device list still needs room
 
     // Upload.
     DeviceContext *ctx = b->ctx;
@@ -448,6 +477,7 @@ int64_t run(dinara_batch *b) {
     DeviceContext *ctx = b->ctx;
     std::lock_guard<std::mutex> serial(ctx->lock);
     DeviceGuard guard(ctx);
+    (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
     const int64_t slots = (int64_t)b->slot_pair.size();
     b->launches = b->first_proven = 0;
     b->run_ms = b->sweep_ms = b->walk_ms = 0;
@@ -723,6 +753,83 @@ int64_t dinara_batch_alignments(dinara_batch *b, dinara_alignment *alignments, i
             statuses[p] = 0;
         }
     });
+    return 0;
+}
+
+int64_t dinara_batch_features(dinara_batch *b, float *features) {
+    if (!b) return DINARA_INVALID_MODE;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    std::fill(features, features + b->pairs * FEATURE_STRIDE, 0.0f);
+    if (slots == 0) return 0;
+    DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
+    DeviceGuard guard(ctx);
+    DeviceBuffer out;
+    if (!out.reserve((size_t)slots * FEATURE_STRIDE * 4)) return DINARA_OUT_OF_MEMORY;
+    SweepArgs a = sweep_args(b);
+    launch_features(a.tape, (int)slots, b->reversed ? 1 : 0, static_cast<float *>(out.ptr), ctx->stream);
+    std::vector<float> host((size_t)slots * FEATURE_STRIDE);
+    if (hipGetLastError() != hipSuccess || !out.read(ctx, host.data(), host.size() * 4)) return DINARA_OUT_OF_MEMORY;
+    for (int64_t s = 0; s < slots; ++s)
+        std::memcpy(features + (int64_t)b->slot_pair[s] * FEATURE_STRIDE, host.data() + s * FEATURE_STRIDE,
+                    FEATURE_STRIDE * 4);
+    return 0;
+}
+
+int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
+    if (!b) return DINARA_INVALID_MODE;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    if (slots == 0) return 0;
+    // The lanes' off-band price (lanes.mojo:333-345) in the batch's costs, as the device's off_band.
+    const CostParams &c = b->costs;
+    auto ins = [&](int64_t k) { return k <= 0 ? 0 : (int64_t)c.ins_open + k * (int64_t)c.ins_ext; };
+    auto del = [&](int64_t k) { return k <= 0 ? 0 : (int64_t)c.del_open + k * (int64_t)c.del_ext; };
+    auto off = [&](int64_t k, int64_t end) {
+        if (k > std::max<int64_t>(end, 0)) return ins(k) + del(k - end);
+        if (k < std::min<int64_t>(end, 0)) return del(-k) + ins(end - k);
+        return int64_t(0);
+    };
+    for (auto &l : b->first_lists) l.clear();
+    for (int64_t s = 0; s < slots; ++s) {
+        const int32_t n = b->rows[s], m = b->cols[s], end = m - n;
+        const int64_t guess = predicted[b->slot_pair[s]];
+        int32_t lo = std::min(0, end) - 1, hi = std::max(0, end) + 1;
+        if (guess >= 0 && !b->user_band) {
+            // The narrowest band whose every outside path costs more than the guess: a pair costing no more than
+            // its guess is proven by it at once.
+            auto widen = [&](int32_t from, int32_t step, int32_t limit) {
+                int32_t k = from, span = 1;
+                while (span < (1 << 22) && k + step * span >= -limit - 1 && k + step * span <= limit + 1 &&
+                       off(k + step * span, end) <= guess)
+                    span *= 2;
+                for (; span > 0; span /= 2)
+                    if (off(k + step * span, end) <= guess) k += step * span;
+                return k;
+            };
+            hi = std::min(std::max(hi, widen(std::max(0, end), 1, m) + 1), m + 1);
+            lo = std::max(std::min(lo, widen(std::min(0, end), -1, n) - 1), -n - 1);
+        }
+        b->first_low[s] = lo;
+        b->first_high[s] = hi;
+        const int need = std::max(hi - lo + 1, 32);
+        const int cls = first_class_for(need, !b->user_band && b->groups_fit, b->block_mask, b->user_band);
+        if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
+    }
+    std::vector<int32_t> first_all;
+    b->first_offset.assign(CLASSES + 1, 0);
+    for (int c2 = 0; c2 < CLASSES; ++c2) {
+        b->first_offset[c2] = (int64_t)first_all.size();
+        first_all.insert(first_all.end(), b->first_lists[c2].begin(), b->first_lists[c2].end());
+    }
+    b->first_offset[CLASSES] = (int64_t)first_all.size();
+    if (first_all.empty()) first_all.push_back(0);  // every pair too wide: the device list still needs room
+    DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
+    DeviceGuard guard(ctx);
+    if (!b->first_low_d.write(ctx, b->first_low.data(), slots * 4) ||
+        !b->first_high_d.write(ctx, b->first_high.data(), slots * 4) ||
+        !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4))
+        return DINARA_OUT_OF_MEMORY;
     return 0;
 }
 

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -25,6 +26,16 @@ using namespace dinara;
 namespace {
 
 constexpr uint8_t UNKNOWN_LETTER = 0xFF;
+
+// A device step that failed: DINARA_OUT_OF_MEMORY, and with DINARA_DEBUG set in the environment a line on stderr
+// naming the step and the runtime's last error.
+int64_t failed(const char *step) {
+    if (std::getenv("DINARA_DEBUG")) {
+        const hipError_t e = hipGetLastError();
+        std::fprintf(stderr, "dinara: %s failed (%s)\n", step, hipGetErrorString(e));
+    }
+    return DINARA_OUT_OF_MEMORY;
+}
 constexpr int64_t INT32_TOP = 0x7fffffff;
 
 // A Scoring checked as the reference checks one (scoring.mojo:200-214, gotoh.mojo:47-54): 1 to 32 letters, no
@@ -203,7 +214,7 @@ int64_t run_pass(Batch *b, int mode, int reversed, const std::vector<int32_t> &s
     }
     at[SCORE_MAX_C + 1] = all.size();
     if (all.empty()) return 0;
-    if (!b->list.upload(ctx, all.data(), all.size() * 4)) return DINARA_OUT_OF_MEMORY;
+    if (!b->list.upload(ctx, all.data(), all.size() * 4)) return failed("list upload");
     const int32_t *list = static_cast<const int32_t *>(b->list.ptr);
     ScoreArgs a = score_args(b, mode, reversed);
     hipEventRecord(start, ctx->stream);
@@ -225,24 +236,24 @@ int64_t run_pass(Batch *b, int mode, int reversed, const std::vector<int32_t> &s
             a.count = (int)std::min(chunk, count - first);
             a.carry = nullptr;
             if (strips > 1) {
-                if (!b->carry.reserve((size_t)a.count * a.carry_stride * 4)) return DINARA_OUT_OF_MEMORY;
+                if (!b->carry.reserve((size_t)a.count * a.carry_stride * 4)) return failed("carry");
                 a.carry = static_cast<int32_t *>(b->carry.ptr);
             }
             if (mode == SCORE_TRACE) {
-                if (!b->arena.reserve((size_t)a.count * a.arena_stride * 4)) return DINARA_OUT_OF_MEMORY;
+                if (!b->arena.reserve((size_t)a.count * a.arena_stride * 4)) return failed("arena");
                 a.arena = static_cast<uint32_t *>(b->arena.ptr);
             }
             launch_score(a, lds, ctx->stream);
-            if (hipGetLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
+            if (hipPeekAtLastError() != hipSuccess) return failed("score launch");
             b->launches += 1;
             if (mode == SCORE_TRACE) {
                 launch_score_walk(a, ctx->stream);
-                if (hipGetLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
+                if (hipPeekAtLastError() != hipSuccess) return failed("walk launch");
             }
         }
     }
     hipEventRecord(stop, ctx->stream);
-    if (hipEventSynchronize(stop) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    if (hipEventSynchronize(stop) != hipSuccess) return failed("pass");
     float ms = 0;
     hipEventElapsedTime(&ms, start, stop);
     *pass_ms += ms;
@@ -297,7 +308,7 @@ int64_t prepare(Batch *b, const char *const *refs, const int64_t *ref_lens, cons
         return b->rows[x] * b->cols[x] > b->rows[y] * b->cols[y];
     });
     const int64_t slots = (int64_t)b->slot_pair.size();
-    std::vector<int64_t> ref_at(slots), qry_at(slots), op_base(slots + 1);
+    std::vector<int64_t> ref_at(std::max<int64_t>(slots, 1)), qry_at(std::max<int64_t>(slots, 1)), op_base(slots + 1);
     int64_t total = 0, ops = 0, text = 0;
     b->max_rows = b->max_cols = 0;
     for (int64_t s = 0; s < slots; ++s) {
@@ -324,8 +335,8 @@ int64_t prepare(Batch *b, const char *const *refs, const int64_t *ref_lens, cons
     b->view_h.assign(4 * std::max<int64_t>(slots, 1), 0);
     const size_t n4 = (size_t)std::max<int64_t>(slots, 1) * 4;
     if (!b->codes.upload(ctx, codes.data(), codes.size()) ||
-        !b->ref_at.upload(ctx, ref_at.data(), std::max<size_t>(ref_at.size(), 1) * 8) ||
-        !b->qry_at.upload(ctx, qry_at.data(), std::max<size_t>(qry_at.size(), 1) * 8) ||
+        !b->ref_at.upload(ctx, ref_at.data(), ref_at.size() * 8) ||
+        !b->qry_at.upload(ctx, qry_at.data(), qry_at.size() * 8) ||
         !b->table.upload(ctx, b->scoring.table, sizeof b->scoring.table) || !b->view.reserve(4 * n4) ||
         !b->score.reserve(n4) || !b->end_row.reserve(n4) || !b->end_col.reserve(n4) || !b->target.reserve(n4))
         return DINARA_OUT_OF_MEMORY;
@@ -365,12 +376,13 @@ int64_t run(Batch *b) {
     const size_t slots = b->slot_pair.size();
     b->ran = true;
     if (slots == 0) return 0;
+    (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
     hipEvent_t start = ctx->event(0), stop = ctx->event(1), begin = ctx->event(2), end = ctx->event(3);
     hipEventRecord(begin, ctx->stream);
     std::vector<int32_t> all(slots);
     for (size_t s = 0; s < slots; ++s) all[s] = (int32_t)s;
     whole_views(b);
-    if (!b->view.write(ctx, b->view_h.data(), b->view_h.size() * 4)) return DINARA_OUT_OF_MEMORY;
+    if (!b->view.write(ctx, b->view_h.data(), b->view_h.size() * 4)) return failed("views");
     const bool align = b->kind == DINARA_BATCH_ALIGNMENTS;
     if (align) hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream);
     int64_t code = 0;
@@ -384,7 +396,7 @@ int64_t run(Batch *b) {
             // vector_score.mojo:37-162), then the span between start and end aligned globally.
             if (!read_slots(b, b->score, b->score_h) || !read_slots(b, b->end_row, b->end_row_h) ||
                 !read_slots(b, b->end_col, b->end_col_h))
-                return DINARA_OUT_OF_MEMORY;
+                return failed("local ends");
             std::vector<int32_t> reaching;
             for (size_t s = 0; s < slots; ++s) {
                 int32_t *v = &b->view_h[4 * s];
@@ -400,11 +412,11 @@ int64_t run(Batch *b) {
             }
             if (!b->view.write(ctx, b->view_h.data(), b->view_h.size() * 4) ||
                 !b->target.write(ctx, b->score_h.data(), slots * 4))
-                return DINARA_OUT_OF_MEMORY;
+                return failed("reach views");
             code = run_pass(b, SCORE_REACH, 1, reaching, start, stop, &b->pass_ms[2], &b->walk_ms);
             if (code == 0) {
                 if (!read_slots(b, b->end_row, b->reach_row_h) || !read_slots(b, b->end_col, b->reach_col_h))
-                    return DINARA_OUT_OF_MEMORY;
+                    return failed("reach starts");
                 for (int32_t s : reaching) {
                     int32_t *v = &b->view_h[4 * s];
                     const int32_t er = b->end_row_h[s], ec = b->end_col_h[s];
@@ -415,13 +427,13 @@ int64_t run(Batch *b) {
                     v[2] = sc;
                     v[3] = ec - sc;
                 }
-                if (!b->view.write(ctx, b->view_h.data(), b->view_h.size() * 4)) return DINARA_OUT_OF_MEMORY;
+                if (!b->view.write(ctx, b->view_h.data(), b->view_h.size() * 4)) return failed("trace views");
                 code = run_pass(b, SCORE_TRACE, 0, reaching, start, stop, &b->pass_ms[3], &b->walk_ms);
             }
         }
     }
     hipEventRecord(end, ctx->stream);
-    if (hipEventSynchronize(end) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    if (hipEventSynchronize(end) != hipSuccess) return failed("run");
     float ms = 0;
     hipEventElapsedTime(&ms, begin, end);
     b->run_ms = ms;
@@ -431,7 +443,7 @@ int64_t run(Batch *b) {
 int64_t fetch(Batch *b) {
     const size_t slots = b->slot_pair.size();
     if (slots == 0) return 0;
-    if (!read_slots(b, b->score, b->score_h)) return DINARA_OUT_OF_MEMORY;
+    if (!read_slots(b, b->score, b->score_h)) return failed("scores");
     if (b->kind == DINARA_BATCH_ALIGNMENTS) {
         uint64_t used = 0;
         b->cigar_offset_h.resize(slots);
@@ -439,9 +451,14 @@ int64_t fetch(Batch *b) {
         if (!b->cigar_used.read(b->ctx, &used, 8) ||
             !b->cigar_offset.read(b->ctx, b->cigar_offset_h.data(), slots * 8) ||
             !b->cigar_length.read(b->ctx, b->cigar_length_h.data(), slots * 4))
-            return DINARA_OUT_OF_MEMORY;
+            return failed("cigar offsets");
         b->cigars_h.resize(std::max<uint64_t>(used, 1));
-        if (used > 0 && !b->cigars.read(b->ctx, b->cigars_h.data(), used)) return DINARA_OUT_OF_MEMORY;
+        if (used > b->cigars.size) {
+            if (std::getenv("DINARA_DEBUG"))
+                std::fprintf(stderr, "dinara: CIGARs claim %llu bytes of %zu\n", (unsigned long long)used, b->cigars.size);
+            return DINARA_OUT_OF_MEMORY;
+        }
+        if (used > 0 && !b->cigars.read(b->ctx, b->cigars_h.data(), used)) return failed("cigars");
     }
     return 0;
 }

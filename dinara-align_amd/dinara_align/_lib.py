@@ -117,6 +117,27 @@ def lib() -> ctypes.CDLL:
     handle.dinara_scoring_batch_free.restype = None
     handle.dinara_scoring_batch_stat.argtypes = [vp, i64]
     handle.dinara_scoring_batch_stat.restype = ctypes.c_double
+    handle.dinara_batch_features.argtypes = [vp, vp]
+    handle.dinara_batch_features.restype = i64
+    handle.dinara_batch_seed.argtypes = [vp, vp]
+    handle.dinara_batch_seed.restype = i64
+    handle.dinara_mlp_new.argtypes = [i64, ctypes.c_uint64]
+    handle.dinara_mlp_new.restype = vp
+    handle.dinara_mlp_free.argtypes = [vp]
+    handle.dinara_mlp_free.restype = None
+    handle.dinara_mlp_parameter_count.argtypes = [vp]
+    handle.dinara_mlp_parameter_count.restype = i64
+    handle.dinara_mlp_get_weights.argtypes = [vp, vp]
+    handle.dinara_mlp_get_weights.restype = i64
+    handle.dinara_mlp_set_weights.argtypes = [vp, vp]
+    handle.dinara_mlp_set_weights.restype = i64
+    handle.dinara_mlp_gradients.argtypes = [vp, i64, vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+    handle.dinara_mlp_gradients.restype = i64
+    handle.dinara_mlp_train.argtypes = [vp, i64, vp, vp, i64, i64, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_double)]
+    handle.dinara_mlp_train.restype = i64
+    handle.dinara_mlp_predict.argtypes = [vp, i64, vp, vp]
+    handle.dinara_mlp_predict.restype = i64
     _LIB = handle
     return handle
 

@@ -204,6 +204,41 @@ void dinara_batch_free(dinara_batch *batch);
  * the pairs it swept. For roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
+/*
+ * New: the learned band proposal. No counterpart in the reference. A small network (288 inputs: 256 4-mer count
+ * differences, reference less query, 2 log-lengths and zero padding; two ReLU layers of 256; one output, log(1 +
+ * cost)) trained and run on the device by bf16 MFMA kernels with fp32 accumulation, whose predicted cost seeds a
+ * batch's first bands. The band certificate still proves every result: a seed only moves work, never an answer.
+ */
+#define DINARA_FEATURE_STRIDE 288
+
+/* Every pair's features, DINARA_FEATURE_STRIDE floats a pair in the caller's order, computed on the device from the
+ * batch's packed letters. */
+int64_t dinara_batch_features(dinara_batch *batch, float *features);
+
+/* Seeds each pair's first band from predicted[i], its guessed cost (below zero: no guess): the narrowest band every
+ * path outside which costs more than the guess, so a pair costing no more is proven in one sweep. Call before
+ * dinara_batch_run; the results do not change, only the work. */
+int64_t dinara_batch_seed(dinara_batch *batch, const int64_t *predicted);
+
+typedef struct dinara_mlp dinara_mlp;
+
+dinara_mlp *dinara_mlp_new(int64_t gpu, uint64_t seed);
+void dinara_mlp_free(dinara_mlp *net);
+/* The fp32 master weights, flat: w1 (288 x 256, row k for input k), b1 (256), w2 (256 x 256), b2 (256), w3 (256),
+ * b3 (1). */
+int64_t dinara_mlp_parameter_count(dinara_mlp *net);
+int64_t dinara_mlp_get_weights(dinara_mlp *net, float *weights);
+int64_t dinara_mlp_set_weights(dinara_mlp *net, const float *weights);
+/* The mean squared error's gradients over `rows` rows at the current weights (flat, as the weights). */
+int64_t dinara_mlp_gradients(dinara_mlp *net, int64_t rows, const float *features, const float *targets,
+                             float *gradients, double *loss);
+/* Adam over `epochs` shuffled passes in batches of `batch` rows (a multiple of 32); the last epoch's mean loss. */
+int64_t dinara_mlp_train(dinara_mlp *net, int64_t rows, const float *features, const float *targets, int64_t epochs,
+                         int64_t batch, double lr, uint64_t seed, double *final_loss);
+/* Predictions, log(1 + cost), for `rows` rows. */
+int64_t dinara_mlp_predict(dinara_mlp *net, int64_t rows, const float *features, float *out);
+
 #ifdef __cplusplus
 }
 #endif
