@@ -38,8 +38,12 @@ VALU_PEAK_TOPS = 39.3  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, one int32 op a 
 HBM_PEAK_GBS = 8000.0
 OPS_PER_CELL = 12  # a gap-affine cell's 7 integer ops for its score and 5 for its decision: the counting convention
 CLASS_NAMES = ["sweep_group<1>", "sweep_group<2>", "sweep_group<4>", "sweep_group<8>", "sweep_wave<1>",
-               "sweep_wave<2>", "sweep_wave<4>", "sweep_wave<8>", "sweep_wave<16>", "sweep_wave<32>"]
-CLASS_WIDTHS = [32, 64, 128, 256, 64, 128, 256, 512, 1024, 2048]
+               "sweep_wave<2>", "sweep_wave<4>", "sweep_wave<8>", "sweep_wave<16>", "sweep_wave<32>",
+               "sweep_block<4>x1", "sweep_block<8>x1", "sweep_block<4>x3", "sweep_block<8>x2", "sweep_block<8>x3",
+               "sweep_block<8>x4", "sweep_block<8>x5", "sweep_block<8>x6", "sweep_block<8>x7", "sweep_block<8>x8",
+               "sweep_block<12>x8", "sweep_block<16>x8"]
+CLASS_WIDTHS = [32, 64, 128, 256, 64, 128, 256, 512, 1024, 2048, 512, 1024, 1536, 2048, 3072, 4096, 5120, 6144, 7168,
+                8192, 12288, 16384]
 CLASSES = len(CLASS_NAMES)
 
 
@@ -120,6 +124,230 @@ def load_baseline(cores: int):
     lib.baseline_alignments.argtypes = [i64, vp, vp, vp, vp, i64, i64, i64, i64, vp, vp, i64]
     lib.baseline_alignments.restype = i64
     return lib, flags
+
+
+def _sum_min_const_minus(P, Q, a, b):
+    """sum over k = a..b of min(P, Q - k), elementwise (0 for an empty range)."""
+    s = Q - P
+    n1 = np.maximum(0, np.minimum(b, s) - a + 1)
+    lo2 = np.maximum(a, s + 1)
+    n2 = np.maximum(0, b - lo2 + 1)
+    return np.where(b >= a, n1 * P + n2 * Q - np.where(n2 > 0, (lo2 + b) * n2 // 2, 0), 0)
+
+
+def _sum_min_plus_const(P, R, a, b):
+    """sum over k = a..b of min(P + k, R), elementwise."""
+    s = R - P
+    hi1 = np.minimum(b, s - 1)
+    n1 = np.maximum(0, hi1 - a + 1)
+    lo2 = np.maximum(a, s)
+    n2 = np.maximum(0, b - lo2 + 1)
+    return np.where(b >= a, n1 * P + np.where(n1 > 0, (a + hi1) * n1 // 2, 0) + n2 * R, 0)
+
+
+def band_cells(n, m, low, high):
+    """Cells (r, c) of an n x m matrix, 1 <= r <= n, 1 <= c <= m, on diagonals low <= c - r <= high (closed form:
+    a diagonal k >= 0 holds min(n, m - k) cells, k < 0 min(n + k, m))."""
+    n = n.astype(np.int64)
+    m = m.astype(np.int64)
+    lo = np.maximum(low, -(n - 1))
+    hi = np.minimum(high, m - 1)
+    a = np.maximum(lo, 0)
+    pos = np.where(hi >= a, _sum_min_const_minus(n, m, a, hi), 0)
+    b2 = np.minimum(hi, -1)
+    neg = np.where(b2 >= lo, _sum_min_plus_const(n, m, lo, b2), 0)
+    return pos + neg
+
+
+def proven_band_cells(n, m, cost, strict):
+    """The cells of the narrowest band the certificate proves at each pair's true cost under (4, 6, 2): out from
+    the start and end diagonals while a path off the band could cost less (no more, `strict`) than the pair."""
+    n = n.astype(np.int64)
+    m = m.astype(np.int64)
+    cost = cost.astype(np.int64)
+    end = m - n
+
+    def off(k):
+        up = k > np.maximum(0, end)
+        down = k < np.minimum(0, end)
+        gap = lambda x: np.where(x > 0, OPEN + x * EXTEND, 0)
+        return np.where(up, gap(k) + gap(k - end), np.where(down, gap(-k) + gap(end - k), 0))
+
+    def grow(start, step, limit):
+        # The furthest k from `start` (by `step`) whose off-band price still admits the pair, binary search.
+        span = np.zeros_like(start)
+        bit = 1 << 16
+        while bit:
+            trial = span + bit
+            k = start + step * trial
+            inside = (trial <= limit) & ((off(k) <= cost) if strict else (off(k) < cost))
+            span = np.where(inside, trial, span)
+            bit >>= 1
+        return start + step * span
+
+    high = grow(np.maximum(0, end), 1, m + 1)
+    low = grow(np.minimum(0, end), -1, n + 1)
+    return band_cells(n, m, low, high)
+
+
+def run_learned(args) -> int:
+    """Gap-affine (4, 6, 2) distances of uneven pairs: the plain path (each pair's first band its start and end
+    diagonals, then the band its found cost proves) against the learned path (the network trained on the aligner's
+    own costs seeds each pair's first band), on the same pairs in the same run, compared byte for byte."""
+    train_n = args.train or 1_000_000
+    eval_n = args.pairs or 200_000
+    cores = host_cores()
+    workers = min(cores, 16)
+    t0 = time.time()
+    train_r, train_q = synthetic.uneven_parallel(train_n, seed=41, workers=workers)
+    eval_r, eval_q = synthetic.uneven_parallel(eval_n, seed=42, workers=workers)
+    print(f"[bench] uneven workload ready: {train_n} training, {eval_n} evaluation pairs in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    handle = _lib.lib()
+    if handle.dinara_device_count() < 1:
+        print("[bench] no GPU visible", file=sys.stderr)
+        return 1
+    costs = _lib.CCosts(MISMATCH, OPEN, EXTEND, -1, 0, 0, 0, -1, 0)
+    F = 288
+
+    def batch_of(refs, qrys):
+        R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+        status = ctypes.c_int64(0)
+        b = handle.dinara_batch_new(0, _lib.BATCH_DISTANCES, len(refs), *R.args(), *Q.args(), ctypes.byref(costs),
+                                    None, None, workers, ctypes.byref(status))
+        if not b:
+            raise RuntimeError(f"dinara_batch_new failed: {status.value}")
+        return b, (R, Q)
+
+    def distances(b, count):
+        out = (ctypes.c_int64 * count)()
+        if handle.dinara_batch_distances(b, out) != 0:
+            raise RuntimeError("distances failed")
+        return np.array(out[:], dtype=np.int64)
+
+    def features(b, count):
+        x = np.zeros((count, F), dtype=np.float32)
+        if handle.dinara_batch_features(b, x.ctypes.data) != 0:
+            raise RuntimeError("features failed")
+        return x
+
+    # The exact aligner labels its own training pairs.
+    t0 = time.time()
+    tb, keep = batch_of(train_r, train_q)
+    if handle.dinara_batch_run(tb) != 0:
+        raise RuntimeError("labelling run failed")
+    labels = distances(tb, train_n)
+    x_train = features(tb, train_n)
+    handle.dinara_batch_free(tb)
+    del keep, train_r, train_q
+    y_train = np.log1p(np.maximum(labels, 0)).astype(np.float32)
+    label_s = time.time() - t0
+    net = handle.dinara_mlp_new(0, 17)
+    loss = ctypes.c_double(0)
+    t0 = time.time()
+    epochs = args.epochs or 3
+    if handle.dinara_mlp_train(net, train_n, x_train.ctypes.data, y_train.ctypes.data, epochs, 4096, 2e-3, 7,
+                               ctypes.byref(loss)) != 0:
+        raise RuntimeError("training failed")
+    train_s = time.time() - t0
+    print(f"[bench] labelled in {label_s:.1f}s, trained {epochs} epochs in {train_s:.1f}s, loss {loss.value:.4f}",
+          file=sys.stderr, flush=True)
+    del x_train
+
+    def timed(b):
+        for _ in range(args.warmup):
+            if handle.dinara_batch_run(b) != 0:
+                raise RuntimeError("warm-up failed")
+        walls, dev, cls = [], [], np.zeros(CLASSES)
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            if handle.dinara_batch_run(b) != 0:
+                raise RuntimeError("run failed")
+            walls.append(time.perf_counter() - t)
+            dev.append(handle.dinara_batch_stat(b, 0))
+            cls += np.array([handle.dinara_batch_stat(b, 100 + c) for c in range(CLASSES)])
+        launches = np.array([handle.dinara_batch_stat(b, 200 + c) for c in range(CLASSES)])
+        first = handle.dinara_batch_stat(b, 4)
+        return float(np.mean(walls) * 1e3), float(np.mean(dev)), cls / args.steps, launches, first
+
+    def path(doubling, margin):
+        b, keep = batch_of(eval_r, eval_q)
+        handle.dinara_batch_set_retry(b, 1 if doubling else 0)
+        seed_ms = 0.0
+        if margin is not None:
+            t = time.perf_counter()
+            if handle.dinara_batch_seed_learned(b, net, margin) != 0:
+                raise RuntimeError("seeding failed")
+            seed_ms = (time.perf_counter() - t) * 1e3
+        ms, dev, cls, launches, first = timed(b)
+        out = distances(b, eval_n)
+        handle.dinara_batch_free(b)
+        del keep
+        return {"ms": ms, "dev": dev, "cls": cls, "launches": launches, "first": first, "seed_ms": seed_ms,
+                "out": out}
+
+    doubling = path(True, None)
+    certified = path(False, None)
+    tried = {margin: path(False, margin) for margin in LEARNED_MARGINS}
+    handle.dinara_mlp_free(net)
+    best_margin = min(tried, key=lambda k: tried[k]["ms"] + tried[k]["seed_ms"])
+    learned = tried[best_margin]
+    exact = float(np.mean([np.array_equal(doubling["out"], r["out"]) for r in [certified, *tried.values()]]))
+    n = np.fromiter((len(r) for r in eval_r), dtype=np.int64, count=eval_n)
+    m = np.fromiter((len(q) for q in eval_q), dtype=np.int64, count=eval_n)
+    cells = proven_band_cells(n, m, np.maximum(learned["out"], 0), strict=False)
+    learned_cls, learned_launches = learned["cls"], learned["launches"]
+    dominant = int(np.argmax(learned_cls))
+    launch_ms = learned_cls[dominant] / max(learned_launches[dominant], 1)
+    achieved = float(cells.sum()) * 7 / (float(learned_cls.sum()) / 1e3) / 1e12
+    roofline = {"bound": "valu", "kernel": CLASS_NAMES[dominant], "achieved": round(achieved, 3),
+                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "launch_ms": round(float(launch_ms), 4),
+                "note": "proven-band cells at the true cost x 7 int ops (distances keep no decisions), over every "
+                        "sweep kernel's time of the learned run"}
+    learned_total = learned["ms"] + learned["seed_ms"]
+    line = {
+        "metric": "pairs/s (gap-affine distances, uneven lengths): learned band vs plain band doubling, bit-exact",
+        "value": round(eval_n / (learned_total / 1e3), 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(learned_total, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"gap-affine (4,6,2) distances, {eval_n} pairs log-normal around 1.5 kbp (sigma 1, "
+                               f"100 bp..20 kbp), 1-15% error (50/25/25); network trained on {train_n} pairs "
+                               "labelled by the aligner", "pairs": eval_n, "train_pairs": train_n,
+                   "epochs": epochs, "margin": best_margin},
+        "plain_doubling": {"value": round(eval_n / (doubling["ms"] / 1e3), 1), "ms_per_step": round(doubling["ms"], 3),
+                           "first_band_proved": round(doubling["first"] / eval_n, 4)},
+        "plain_certified": {"value": round(eval_n / (certified["ms"] / 1e3), 1),
+                            "ms_per_step": round(certified["ms"], 3),
+                            "first_band_proved": round(certified["first"] / eval_n, 4)},
+        "learned": {"ms_per_step": round(learned["ms"], 3), "seed_ms": round(learned["seed_ms"], 3),
+                    "first_band_proved": round(learned["first"] / eval_n, 4),
+                    "by_margin_ms": {str(k): round(v["ms"] + v["seed_ms"], 3) for k, v in tried.items()}},
+        "speedup_vs_plain_doubling": round(doubling["ms"] / learned_total, 3),
+        "speedup_vs_plain_certified": round(certified["ms"] / learned_total, 3),
+        "gcups": round(float((n * m).sum()) / (learned_total / 1e3) / 1e9, 2),
+        "bit_exact_vs_plain": exact,
+        "training": {"label_s": round(label_s, 2), "train_s": round(train_s, 2), "final_loss": round(loss.value, 5)},
+        "roofline": roofline,
+        "cpu_baseline": None,
+        "kernels_ms_per_step": {name: {CLASS_NAMES[c]: round(float(r["cls"][c]), 3) for c in range(CLASSES)
+                                       if r["cls"][c] > 0}
+                                for name, r in (("doubling", doubling), ("certified", certified),
+                                                ("learned", learned))},
+    }
+    print(json.dumps(line))
+    return 0
+
+
+LEARNED_MARGINS = (-0.05, 0.0, 0.05, 0.1)  # seeds e^(prediction + margin) - 1: room, or not, for the network's error
 
 
 LOCAL_OPS_PER_CELL = 10  # a local cell: 4 adds, 5 maxes and the floor at zero, the counting convention
@@ -261,9 +489,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=None, help="pairs (default: the workload's own count)")
-    ap.add_argument("--workload", choices=("affine", "local"), default="affine",
+    ap.add_argument("--workload", choices=("affine", "local", "learned"), default="affine",
                     help="affine: global gap-affine (4,6,2) CIGARs, 100k x 250 bp (the default); local: local "
-                         "alignment under a 20-letter table, 50k x (3000 aa window, 300 aa query)")
+                         "alignment under a 20-letter table, 50k x (3000 aa window, 300 aa query); learned: gap-affine "
+                         "distances of 200k uneven pairs, the learned band against plain band doubling")
+    ap.add_argument("--train", type=int, default=None, help="learned: training pairs (default 1,000,000)")
+    ap.add_argument("--epochs", type=int, default=None, help="learned: training epochs (default 3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -272,6 +503,8 @@ def main() -> int:
         return 2
     if args.workload == "local":
         return run_local(args)
+    if args.workload == "learned":
+        return run_learned(args)
     args.pairs = args.pairs or 100_000
 
     t0 = time.time()

@@ -28,7 +28,8 @@ __device__ __forceinline__ int64_t off_band(const CostParams &c, int64_t k, int6
 }
 
 // Whether `found`, the least cost over lane diagonals lo..hi, settles the pair (reference lanes.mojo:1360-1392);
-// if not, the band every cheaper (strict: no dearer) path could visit, in *want_low..*want_high.
+// if not, the band every cheaper (strict: no dearer) path could visit, in *want_low..*want_high (or, under the
+// doubling policy, the same band twice as wide).
 __device__ __forceinline__ bool certify(const SweepArgs &a, int rows, int cols, int64_t found, int lo, int hi, int *want_low,
                         int *want_high) {
     const int end = cols - rows;
@@ -45,6 +46,13 @@ __device__ __forceinline__ bool certify(const SweepArgs &a, int rows, int cols, 
     const int64_t off = beyond_high < beyond_low ? beyond_high : beyond_low;
     const bool beaten = a.strict ? found < off : found <= off;
     if (beaten || off > a.cap) return true;
+    if (a.doubling) {
+        // Plain band doubling (the policy the learned band is measured against): the same band twice as wide.
+        const int half = (hi - lo + 1) / 2 + 1;
+        *want_low = lo - half > bottom ? lo - half : bottom;
+        *want_high = hi + half < top ? hi + half : top;
+        return false;
+    }
     const int64_t capped = found < a.cap ? found : a.cap;
     const int64_t target = a.strict ? capped + 1 : (found <= a.cap ? found : a.cap + 1);
     int h = hi, l = lo;
@@ -55,11 +63,12 @@ __device__ __forceinline__ bool certify(const SweepArgs &a, int rows, int cols, 
     return false;
 }
 
-// The class of band a pair needing `need` diagonals takes next: the narrowest that holds them. A group or block
-// band is placed on an even diagonal, so it holds one diagonal fewer than its width. Pairs under a user band take
-// the wave bands, which bar diagonals outside it.
-__device__ __forceinline__ int class_for(int need, const SweepArgs &a) {
-    if (!a.user_band && a.groups_fit) {
+// The class of band a pair of `letters` letters needing `need` diagonals takes next: the narrowest that holds them.
+// A group or block band is placed on an even diagonal, so it holds one diagonal fewer than its width. Pairs under a
+// user band take the wave bands, which bar diagonals outside it; pairs longer than a group launch stages take
+// blocks.
+__device__ __forceinline__ int class_for(int need, int letters, const SweepArgs &a) {
+    if (!a.user_band && a.groups_fit && letters <= a.group_max_letters) {
         for (int cls = CLASS_GROUP32; cls <= CLASS_GROUP256; ++cls)
             if (need < band_width(cls)) return cls;
     }
@@ -75,7 +84,7 @@ __device__ __forceinline__ int class_for(int need, const SweepArgs &a) {
 __device__ __forceinline__ void file_retry(const SweepArgs &a, int slot, int letters, int want_low, int want_high) {
     a.want_low[slot] = want_low;
     a.want_high[slot] = want_high;
-    const int cls = class_for(want_high - want_low + 1, a);
+    const int cls = class_for(want_high - want_low + 1, letters, a);
     // A wave sweep stages both sequences in LDS: a pair they overflow has no class to go to.
     if (cls == CLASS_TOO_WIDE || (band_is_wave(cls) && letters > a.lds_limit)) {
         a.status[slot] = 2;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <functional>
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -121,7 +122,7 @@ int64_t pair_code(const char *ref, int64_t n, const char *qry, int64_t m) {
 // a group band starts on an even diagonal), else the narrowest block band, else the narrowest wave band (as the
 // device's class_for).
 int first_class_for(int need, bool groups, int block_mask, bool user_band) {
-    if (groups)
+    if (groups && !user_band)
         for (int cls = CLASS_GROUP32; cls <= CLASS_GROUP256; ++cls)
             if (need < class_width(cls)) return cls;
     if (!user_band)
@@ -160,11 +161,13 @@ struct dinara_batch {
     std::vector<int64_t> scratch_base;
     int bits = 2;
     bool reversed = false;                // the tape holds both sequences back to front (Ties.LEFT)
+    bool doubling = false;                // retries double the band rather than take the band the cost proves
     size_t lds_limit = 0;
     DeviceContext *ctx = nullptr;
     int max_letters = 0;                  // the most letters of a pair, both sequences
     int32_t max_rows = 0;
     int group_letters = 4;                // LDS bytes a pair takes in a group launch
+    int group_max_letters = 0;            // the longest pair a group launch stages
     bool groups_fit = false;              // four pairs' letters fit one block's LDS, their costs the tags
     int block_mask = 0;                   // the block classes whose letters and tags fit
     int64_t dearest = 1;                  // the dearest single step: a substitution or a gap's first letter
@@ -310,17 +313,22 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         b->max_letters = std::max<int>(b->max_letters, b->rows[s] + b->cols[s]);
     }
     // A group pair's LDS region: both sequences and room for reads up to half the widest group band (256) past
-    // the query's end (group.hip).
-    b->group_letters = (b->max_letters + 128 + 16 + 3) & ~3;
+    // the query's end (group.hip). Pairs up to a quarter of the LDS take group bands; longer ones take blocks.
+    const int group_cap = (int)(b->lds_limit / 4) - 128 - 16 - 4;
+    b->group_max_letters = 0;
+    for (int64_t s = 0; s < slots; ++s)
+        if (b->rows[s] + b->cols[s] <= group_cap)
+            b->group_max_letters = std::max(b->group_max_letters, b->rows[s] + b->cols[s]);
+    b->group_letters = (b->group_max_letters + 128 + 16 + 3) & ~3;
     // The group sweep holds costs times 16 with a tag (group.hip) and runs up to half its widest band past the
     // corner: every value it makes stays under 2^30.
-    b->groups_fit = (size_t)4 * b->group_letters <= b->lds_limit &&
-                    ((int64_t)b->max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
+    b->groups_fit = group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
+                    ((int64_t)b->group_max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
     b->block_mask = 0;
     for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES; ++cls)
         if (block_letters(b->max_letters, cls) <= b->lds_limit &&
             ((int64_t)b->max_letters + class_width(cls) + 8) * b->dearest < REAL_LIMIT &&
-            !(cls == CLASSES - 1 && b->kind == DINARA_BATCH_ALIGNMENTS))  // the widest keeps no decisions
+            !(block_s(cls) > 8 && b->kind == DINARA_BATCH_ALIGNMENTS))  // the widest keep no decisions
             b->block_mask |= 1 << (cls - CLASS_BLOCK_FIRST);
     // The first band: the lanes' start and end diagonals and one either side (lanes.mojo:1207-1215), within the
     // user band; swept at no less than the initial width, centred.
@@ -340,7 +348,8 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         b->first_low[s] = lo;
         b->first_high[s] = hi;
         const int need = std::max(hi - lo + 1, b->initial_width);
-        const int cls = first_class_for(need, !b->user_band && b->groups_fit, b->block_mask, b->user_band);
+        const int cls = first_class_for(need, b->groups_fit && n + m <= b->group_max_letters, b->block_mask,
+                                        b->user_band);
         // A pair whose start and end diagonals lie further apart than the widest band is left unswept: its
         // status stays unsettled and it reports DINARA_OUT_OF_MEMORY.
         if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
@@ -417,7 +426,9 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.walk_class = static_cast<int32_t *>(b->walk_class.ptr);
     a.walk_flags = static_cast<uint64_t *>(b->walk_flags.ptr);
     a.group_letters = b->group_letters;
+    a.doubling = b->doubling ? 1 : 0;
     a.groups_fit = b->groups_fit;
+    a.group_max_letters = b->group_max_letters;
     a.block_mask = b->block_mask;
     a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
     a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
@@ -482,12 +493,16 @@ int64_t run(dinara_batch *b) {
     b->launches = b->first_proven = 0;
     b->run_ms = b->sweep_ms = b->walk_ms = 0;
     for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_pairs[c] = 0;
-    hipEventRecord(ctx->start, ctx->stream);
+    if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
     if (slots > 0) {
-        hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream);
-        hipMemcpyAsync(b->want_high.ptr, b->first_high_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream);
-        if (b->kind == DINARA_BATCH_ALIGNMENTS) hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream);
-        hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream);
+        bool ok = hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice,
+                                 ctx->stream) == hipSuccess &&
+                  hipMemcpyAsync(b->want_high.ptr, b->first_high_d.ptr, slots * 4, hipMemcpyDeviceToDevice,
+                                 ctx->stream) == hipSuccess &&
+                  hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream) == hipSuccess;
+        if (b->kind == DINARA_BATCH_ALIGNMENTS)
+            ok = ok && hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream) == hipSuccess;
+        if (!ok) return DINARA_OUT_OF_MEMORY;
     }
     // Round 0 sweeps the first lists (uploaded once); each later round sweeps the lists the round before filed,
     // kept on the device, two sets taking turns. Only the counts come back to the host.
@@ -505,31 +520,31 @@ int64_t run(dinara_batch *b) {
         int32_t *out_lists = lists + (size_t)(round % 2) * CLASSES * capacity;
         int32_t *out_counts = count_d + (round % 2) * CLASSES;
         const int32_t *in_lists = lists + (size_t)((round + 1) % 2) * CLASSES * capacity;
-        hipMemsetAsync(out_counts, 0, CLASSES * 4, ctx->stream);
-        hipEventRecord(ctx->sweep_start, ctx->stream);
+        if (hipMemsetAsync(out_counts, 0, CLASSES * 4, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        (void)hipEventRecord(ctx->sweep_start, ctx->stream);
         for (int c = 0; c < CLASSES; ++c) {
             if (counts[c] == 0) continue;
             const int32_t *list = round == 0 ? static_cast<const int32_t *>(b->first_lists_d.ptr) + b->first_offset[c]
                                              : in_lists + (size_t)c * capacity;
             // Each class's launches between two events of their own, for the per-kernel time.
-            hipEventRecord(ctx->event(2 * c), ctx->stream);
+            (void)hipEventRecord(ctx->event(2 * c), ctx->stream);
             if (const int64_t code = launch_class(b, c, list, counts[c], out_lists, out_counts, arena_index))
                 return code;
-            hipEventRecord(ctx->event(2 * c + 1), ctx->stream);
+            (void)hipEventRecord(ctx->event(2 * c + 1), ctx->stream);
         }
-        hipEventRecord(ctx->sweep_stop, ctx->stream);
+        (void)hipEventRecord(ctx->sweep_stop, ctx->stream);
         int32_t filed[CLASSES];
         if (hipMemcpyAsync(filed, out_counts, sizeof(filed), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             return DINARA_OUT_OF_MEMORY;
         float ms = 0;
-        hipEventElapsedTime(&ms, ctx->sweep_start, ctx->sweep_stop);
+        (void)hipEventElapsedTime(&ms, ctx->sweep_start, ctx->sweep_stop);
         b->sweep_ms += ms;
         int64_t retried = 0;
         for (int c = 0; c < CLASSES; ++c) {
             if (counts[c] > 0) {
                 float class_ms = 0;
-                hipEventElapsedTime(&class_ms, ctx->event(2 * c), ctx->event(2 * c + 1));
+                (void)hipEventElapsedTime(&class_ms, ctx->event(2 * c), ctx->event(2 * c + 1));
                 b->class_ms[c] += class_ms;
             }
             counts[c] = filed[c];
@@ -539,19 +554,19 @@ int64_t run(dinara_batch *b) {
     }
     if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
         SweepArgs a = sweep_args(b);
-        hipEventRecord(ctx->event(2 * CLASSES), ctx->stream);
+        (void)hipEventRecord(ctx->event(2 * CLASSES), ctx->stream);
         launch_walk(a, ctx->stream);
-        hipEventRecord(ctx->event(2 * CLASSES + 1), ctx->stream);
+        (void)hipEventRecord(ctx->event(2 * CLASSES + 1), ctx->stream);
     }
-    hipEventRecord(ctx->stop, ctx->stream);
-    if (hipEventSynchronize(ctx->stop) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    if (hipEventRecord(ctx->stop, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->stop) != hipSuccess)
+        return DINARA_OUT_OF_MEMORY;
     if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
         float walk_ms = 0;
-        hipEventElapsedTime(&walk_ms, ctx->event(2 * CLASSES), ctx->event(2 * CLASSES + 1));
+        (void)hipEventElapsedTime(&walk_ms, ctx->event(2 * CLASSES), ctx->event(2 * CLASSES + 1));
         b->walk_ms = walk_ms;
     }
     float total = 0;
-    hipEventElapsedTime(&total, ctx->start, ctx->stop);
+    (void)hipEventElapsedTime(&total, ctx->start, ctx->stop);
     b->run_ms = total;
     b->ran = true;
     return 0;
@@ -776,8 +791,13 @@ int64_t dinara_batch_features(dinara_batch *b, float *features) {
     return 0;
 }
 
-int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
-    if (!b) return DINARA_INVALID_MODE;
+}  // extern "C"
+
+namespace {
+
+// Every slot's first band from its guessed cost (`guess_of(slot)`, below zero: no guess), classes and device lists.
+template <typename GuessOf>
+int64_t seed_slots(dinara_batch *b, GuessOf guess_of) {
     const int64_t slots = (int64_t)b->slot_pair.size();
     if (slots == 0) return 0;
     // The lanes' off-band price (lanes.mojo:333-345) in the batch's costs, as the device's off_band.
@@ -792,7 +812,7 @@ int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
     for (auto &l : b->first_lists) l.clear();
     for (int64_t s = 0; s < slots; ++s) {
         const int32_t n = b->rows[s], m = b->cols[s], end = m - n;
-        const int64_t guess = predicted[b->slot_pair[s]];
+        const int64_t guess = guess_of(s);
         int32_t lo = std::min(0, end) - 1, hi = std::max(0, end) + 1;
         if (guess >= 0 && !b->user_band) {
             // The narrowest band whose every outside path costs more than the guess: a pair costing no more than
@@ -812,7 +832,8 @@ int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
         b->first_low[s] = lo;
         b->first_high[s] = hi;
         const int need = std::max(hi - lo + 1, 32);
-        const int cls = first_class_for(need, !b->user_band && b->groups_fit, b->block_mask, b->user_band);
+        const int cls = first_class_for(need, b->groups_fit && n + m <= b->group_max_letters, b->block_mask,
+                                        b->user_band);
         if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
     }
     std::vector<int32_t> first_all;
@@ -830,6 +851,45 @@ int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
         !b->first_high_d.write(ctx, b->first_high.data(), slots * 4) ||
         !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4))
         return DINARA_OUT_OF_MEMORY;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
+    if (!b) return DINARA_INVALID_MODE;
+    return seed_slots(b, [&](int64_t s) { return predicted[b->slot_pair[s]]; });
+}
+
+int64_t dinara_batch_seed_learned(dinara_batch *b, dinara_mlp *net, double margin) {
+    if (!b || !net) return DINARA_INVALID_MODE;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    if (slots == 0) return 0;
+    std::vector<float> predicted((size_t)slots);
+    {
+        DeviceContext *ctx = b->ctx;
+        std::lock_guard<std::mutex> serial(ctx->lock);
+        DeviceGuard guard(ctx);
+        DeviceBuffer features;
+        if (!features.reserve((size_t)slots * FEATURE_STRIDE * 4)) return DINARA_OUT_OF_MEMORY;
+        SweepArgs a = sweep_args(b);
+        launch_features(a.tape, (int)slots, b->reversed ? 1 : 0, static_cast<float *>(features.ptr), ctx->stream);
+        if (hipPeekAtLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        if (const int64_t code = mlp_predict_device(net, static_cast<const float *>(features.ptr), slots,
+                                                    predicted.data()))
+            return code;
+    }
+    return seed_slots(b, [&](int64_t s) {
+        const double guess = std::expm1((double)predicted[s] + margin);
+        return guess > 0 ? (int64_t)guess : int64_t(0);
+    });
+}
+
+int64_t dinara_batch_set_retry(dinara_batch *b, int64_t doubling) {
+    if (!b) return DINARA_INVALID_MODE;
+    b->doubling = doubling != 0;
     return 0;
 }
 

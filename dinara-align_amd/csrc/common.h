@@ -33,33 +33,44 @@ struct Tape {
 };
 
 // The sweep classes: four sixteen-lane-group bands (32, 64, 128, 256 diagonals, group.hip); wave-per-pair bands of
-// 64Q (Q = 1..32, sweep.hip), which also take every pair under a user band; and block bands, one pair a block of
-// waves (group.hip): 512 (S = 4, one wave), 1024 to 8192 (S = 8, 1 to 8 waves), 16384 (S = 16, 8 waves).
+// 64Q (Q = 1..32, sweep.hip), which also take every pair under a user band; and block bands, one pair a block of G
+// waves with S diagonal pairs a lane, 128SG diagonals (group.hip): 512 to 16384, in steps of at most a third.
 constexpr int CLASS_GROUP32 = 0;
 constexpr int CLASS_GROUP256 = 3;
 constexpr int CLASS_WAVE_FIRST = 4;  // Q = 1, 2, 4, 8, 16, 32 for classes 4..9
 constexpr int CLASS_WAVE_LAST = 9;
 constexpr int CLASS_BLOCK_FIRST = 10;
-constexpr int CLASSES = 16;
+constexpr int BLOCK_CLASSES = 12;
+constexpr int CLASSES = CLASS_BLOCK_FIRST + BLOCK_CLASSES;
 constexpr int CLASS_TOO_WIDE = -1;
+
+// A block class's diagonal pairs a lane (S) and waves (G), block class b = cls - CLASS_BLOCK_FIRST: 512 (4 x 1),
+// 1024 (8 x 1), 1536 (4 x 3), 2048 (8 x 2), 3072 (8 x 3), then 8 x 4..8 (4096..8192), 12288 (12 x 8), 16384 (16 x 8).
+__host__ __device__ constexpr int block_s(int cls) {
+    return (cls - CLASS_BLOCK_FIRST == 0 || cls - CLASS_BLOCK_FIRST == 2) ? 4
+           : cls - CLASS_BLOCK_FIRST == 10                              ? 12
+           : cls - CLASS_BLOCK_FIRST == 11                              ? 16
+                                                                        : 8;
+}
+__host__ __device__ constexpr int block_waves(int cls) {
+    return cls - CLASS_BLOCK_FIRST == 0   ? 1
+           : cls - CLASS_BLOCK_FIRST == 1 ? 1
+           : cls - CLASS_BLOCK_FIRST == 2 ? 3
+           : cls - CLASS_BLOCK_FIRST == 3 ? 2
+           : cls - CLASS_BLOCK_FIRST == 4 ? 3
+           : cls - CLASS_BLOCK_FIRST <= 9 ? cls - CLASS_BLOCK_FIRST - 1
+                                          : 8;
+}
 
 // The diagonals a class's band holds.
 __host__ __device__ constexpr int band_width(int cls) {
     return cls <= CLASS_GROUP256    ? 32 << cls
            : cls <= CLASS_WAVE_LAST ? 64 << (cls - CLASS_WAVE_FIRST)
-           : cls == CLASS_BLOCK_FIRST ? 512
-                                      : 1024 << (cls - CLASS_BLOCK_FIRST - 1);
+                                    : 128 * block_s(cls) * block_waves(cls);
 }
 __host__ __device__ constexpr bool band_is_wave(int cls) { return cls >= CLASS_WAVE_FIRST && cls <= CLASS_WAVE_LAST; }
 __host__ __device__ constexpr bool band_is_block(int cls) { return cls >= CLASS_BLOCK_FIRST; }
-// A block class's diagonal pairs a lane and waves a pair: 512 = 4 x 1 wave, 1024 to 8192 = 8 x 1 to 8 waves,
-// 16384 = 16 x 8 waves (a block of at most 8 waves keeps 256 registers a lane).
-__host__ __device__ constexpr int block_s(int cls) {
-    return cls == CLASS_BLOCK_FIRST ? 4 : (cls == CLASSES - 1 ? 16 : 8);
-}
-__host__ __device__ constexpr int block_waves(int cls) {
-    return cls == CLASS_BLOCK_FIRST ? 1 : (cls == CLASSES - 1 ? 8 : 1 << (cls - CLASS_BLOCK_FIRST - 1));
-}
+
 
 // Everything a sweep launch reads and writes.
 struct SweepArgs {
@@ -74,6 +85,7 @@ struct SweepArgs {
     int strict;           // 1 for alignments: no path off the band may cost as little
     int64_t cap;          // the cost cap, or a value past every cost
     int user_band;        // 1 when diagonals outside [user_low, user_high] (lane terms) are barred
+    int doubling;         // 1: an unproven pair's next band is its band twice as wide, not the band its cost proves
     int user_low, user_high;
     int eqx;
     int forward_cigar;    // 1 when the tape holds both sequences reversed (Ties.LEFT): moves come in order
@@ -91,6 +103,7 @@ struct SweepArgs {
     uint64_t *walk_flags;    // per slot: the address of its settled band's decisions
     int group_letters;       // LDS bytes a pair takes in a group launch (four pairs a block)
     int groups_fit;          // 1 when four pairs' letters fit one block's LDS
+    int group_max_letters;   // the longest pair (both sequences) a group launch stages; longer pairs take blocks
     int block_mask;          // bit c - CLASS_BLOCK_FIRST: block class c's letters fit its LDS and its costs its tags
     // CIGARs: each slot's scratch run of 2(n+m)+2 bytes from cigar_scratch_base, compacted into `cigars` at an
     // offset taken from `cigar_used`

@@ -539,7 +539,14 @@ void launch_block(int s, int waves, const SweepArgs &a, size_t lds, hipStream_t 
     switch (s) {
         case 4: DINARA_BLOCK(4) break;
         case 8: DINARA_BLOCK(8) break;
-        case 16:  // distances only: a band this wide keeps no decisions (its record would spill registers)
+        // Distances only past S = 8: a band this wide keeps no decisions (its record would spill registers).
+        case 12:
+            if (!a.record) {
+                if (shifted) hipLaunchKernelGGL((sweep_block<12, false, true>), grid, block, lds, stream, a);
+                else hipLaunchKernelGGL((sweep_block<12, false, false>), grid, block, lds, stream, a);
+            }
+            break;
+        case 16:
             if (!a.record) {
                 if (shifted) hipLaunchKernelGGL((sweep_block<16, false, true>), grid, block, lds, stream, a);
                 else hipLaunchKernelGGL((sweep_block<16, false, false>), grid, block, lds, stream, a);

@@ -17,3 +17,9 @@ constexpr int FEATURE_STRIDE = 288;  // a pair's features padded to whole MFMA s
 void launch_features(const Tape &tape, int slots, int reversed, float *out, hipStream_t stream);
 
 }  // namespace dinara
+
+struct dinara_mlp;
+
+// The network's predictions (log(1 + cost)) for `rows` rows of fp32 features already in device memory, into `out`
+// (host memory). 0, or a DINARA_* code.
+int64_t mlp_predict_device(dinara_mlp *net, const float *features, int64_t rows, float *out);

@@ -282,6 +282,30 @@ bool upload_rows(Net *net, int64_t rows, const float *features, int *padded) {
 
 }  // namespace
 
+int64_t mlp_predict_device(dinara_mlp *net, const float *features, int64_t rows, float *out) {
+    if (!net) return DINARA_INVALID_MODE;
+    if (rows <= 0) return 0;
+    DeviceGuard guard(net->ctx);
+    hipStream_t s = net->ctx->stream;
+    const int64_t chunk = 1 << 16;
+    if (!reserve_rows(net, (int)chunk) || !net->index.reserve((size_t)chunk * 4)) return DINARA_OUT_OF_MEMORY;
+    std::vector<int32_t> idx((size_t)chunk);
+    for (int64_t first = 0; first < rows; first += chunk) {
+        const int64_t count = std::min(chunk, rows - first);
+        const int padded = (int)((count + 31) / 32 * 32);
+        for (int64_t i = 0; i < count; ++i) idx[i] = (int32_t)(first + i);
+        if (!net->index.write(net->ctx, idx.data(), (size_t)count * 4)) return DINARA_OUT_OF_MEMORY;
+        hipLaunchKernelGGL(gather_rows, dim3(padded), dim3(FEATURE_STRIDE), 0, s, features,
+                           static_cast<const int32_t *>(net->index.ptr), (int)count, h16(net->x), padded);
+        forward(net, padded, s);
+        hipLaunchKernelGGL(head, dim3((padded + 255) / 256), dim3(256), 0, s, h16(net->a2), h16(net->bf, net->off_w3),
+                           f32(net->master, net->off_b3), net->hidden, padded, (int)count, nullptr, f32(net->y),
+                           nullptr, nullptr, nullptr, nullptr);
+        if (!net->y.read(net->ctx, out + first, (size_t)count * 4)) return DINARA_OUT_OF_MEMORY;
+    }
+    return 0;
+}
+
 extern "C" {
 
 dinara_mlp *dinara_mlp_new(int64_t gpu, uint64_t seed) {

@@ -251,13 +251,9 @@ void launch_sweep(const SweepArgs &a, size_t lds, hipStream_t stream) {
         case CLASS_WAVE_FIRST + 3: launch_wave<8>(a, lds, stream); break;
         case CLASS_WAVE_FIRST + 4: launch_wave<16>(a, lds, stream); break;
         case CLASS_WAVE_FIRST + 5: launch_wave<32>(a, lds, stream); break;
-        case CLASS_BLOCK_FIRST + 0:
-        case CLASS_BLOCK_FIRST + 1:
-        case CLASS_BLOCK_FIRST + 2:
-        case CLASS_BLOCK_FIRST + 3:
-        case CLASS_BLOCK_FIRST + 4:
-        case CLASS_BLOCK_FIRST + 5: launch_block(block_s(a.cls), block_waves(a.cls), a, lds, stream); break;
-        default: break;
+        default:
+            if (band_is_block(a.cls)) launch_block(block_s(a.cls), block_waves(a.cls), a, lds, stream);
+            break;
     }
 }
 

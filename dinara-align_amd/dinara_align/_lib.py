@@ -121,6 +121,10 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_features.restype = i64
     handle.dinara_batch_seed.argtypes = [vp, vp]
     handle.dinara_batch_seed.restype = i64
+    handle.dinara_batch_seed_learned.argtypes = [vp, vp, ctypes.c_double]
+    handle.dinara_batch_seed_learned.restype = i64
+    handle.dinara_batch_set_retry.argtypes = [vp, i64]
+    handle.dinara_batch_set_retry.restype = i64
     handle.dinara_mlp_new.argtypes = [i64, ctypes.c_uint64]
     handle.dinara_mlp_new.restype = vp
     handle.dinara_mlp_free.argtypes = [vp]

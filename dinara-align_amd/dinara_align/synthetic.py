@@ -30,11 +30,14 @@ def mutate(codes: np.ndarray, owner: np.ndarray, rate: np.ndarray, weights: Tupl
     """Mutates the concatenated sequences `codes` (pair of each letter `owner`, rate per pair `rate`).
     `weights` are substitution, insertion, deletion; `indel_p` > 0 draws indel lengths Geometric(indel_p)."""
     total = len(codes)
-    event = rng.random(total) < rate[owner]
-    kind = rng.choice(3, size=total, p=np.asarray(weights) / np.sum(weights))
-    sub = event & (kind == 0)
-    ins = event & (kind == 1)
-    dele = event & (kind == 2)
+    events = np.flatnonzero(rng.random(total) < rate[owner])
+    kind = rng.choice(3, size=len(events), p=np.asarray(weights) / np.sum(weights))
+    sub = np.zeros(total, dtype=bool)
+    ins = np.zeros(total, dtype=bool)
+    dele = np.zeros(total, dtype=bool)
+    sub[events[kind == 0]] = True
+    ins[events[kind == 1]] = True
+    dele[events[kind == 2]] = True
     out = codes.copy()
     out[sub] = (out[sub] + rng.integers(1, alphabet, size=int(sub.sum()))) % alphabet
     keep = np.ones(total, dtype=bool)
@@ -58,7 +61,7 @@ def mutate(codes: np.ndarray, owner: np.ndarray, rate: np.ndarray, weights: Tupl
 
 
 def _uniform(rng, lengths: np.ndarray, alphabet: int):
-    owner = np.repeat(np.arange(len(lengths)), lengths)
+    owner = np.repeat(np.arange(len(lengths), dtype=np.int32), lengths)
     codes = rng.integers(0, alphabet, size=int(lengths.sum())).astype(np.uint8)
     return codes, owner
 
@@ -90,6 +93,28 @@ def uneven(pairs: int = 200_000, seed: int = 3, sigma: float = 1.0, clip=(100, 2
     codes, owner = _uniform(rng, lengths, 4)
     mutated, mowner = mutate(codes, owner, rates, (0.5, 0.25, 0.25), 4, rng)
     return _split(codes, owner, pairs, DNA), _split(mutated, mowner, pairs, DNA)
+
+
+def _uneven_chunk(job):
+    pairs, seed, clip = job
+    return uneven(pairs, seed=seed, clip=clip)
+
+
+def uneven_parallel(pairs: int, seed: int = 3, clip=(100, 20_000), chunk: int = 50_000, workers: int = 16):
+    """`uneven` in chunks of `chunk` pairs on `workers` processes, chunk i seeded `seed * 1000 + i` (the same pairs
+    for the same arguments whatever the machine)."""
+    jobs = [(min(chunk, pairs - at), seed * 1000 + i, clip) for i, at in enumerate(range(0, pairs, chunk))]
+    if len(jobs) <= 1 or workers <= 1:
+        parts = [_uneven_chunk(j) for j in jobs]
+    else:
+        import multiprocessing
+        with multiprocessing.get_context("fork").Pool(min(workers, len(jobs))) as pool:
+            parts = pool.map(_uneven_chunk, jobs)
+    refs, qrys = [], []
+    for r, q in parts:
+        refs.extend(r)
+        qrys.extend(q)
+    return refs, qrys
 
 
 def protein_table(seed: int = 5) -> Tuple[str, tuple]:

@@ -199,9 +199,10 @@ void dinara_batch_free(dinara_batch *batch);
 
 /* New: what the last dinara_batch_run spent, read from HIP events on the library's stream: `what` 0 the
  * whole run (ms), 1 the banded sweeps (ms), 2 the number of sweep launches, 4 the pairs the first band proved,
- * 5 the walk-back kernel (ms); for band class c (0-3 sixteen lanes a pair, 32/64/128/256 diagonals; 4-9 one
- * wave a pair, 64..2048) 100+c its kernels' time (ms), 200+c its launches, 400+c
- * the pairs it swept. For roofline reporting. No counterpart in the reference. */
+ * 5 the walk-back kernel (ms); for band class c 100+c its kernels' time (ms), 200+c its launches, 400+c the pairs
+ * it swept. Classes: 0-3 sixteen lanes a pair (32/64/128/256 diagonals), 4-9 one wave a pair (64..2048, every
+ * pair under a user band), 10-21 one pair a block of waves (512, 1024, 1536, 2048, 3072, 4096..8192 in steps of
+ * 1024, 12288, 16384; the last two distances only). For roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
 /*
@@ -222,6 +223,14 @@ int64_t dinara_batch_features(dinara_batch *batch, float *features);
 int64_t dinara_batch_seed(dinara_batch *batch, const int64_t *predicted);
 
 typedef struct dinara_mlp dinara_mlp;
+
+/* The same seeds from a trained network without leaving the device: the batch's features, the network's
+ * predictions, and each pair's guess e^(prediction + margin) - 1. */
+int64_t dinara_batch_seed_learned(dinara_batch *batch, dinara_mlp *net, double margin);
+
+/* How an unproven pair's next band is chosen: 0 (the default) the band its found cost proves, 1 plain band doubling,
+ * the same band twice as wide (the policy the learned band is measured against). */
+int64_t dinara_batch_set_retry(dinara_batch *batch, int64_t doubling);
 
 dinara_mlp *dinara_mlp_new(int64_t gpu, uint64_t seed);
 void dinara_mlp_free(dinara_mlp *net);

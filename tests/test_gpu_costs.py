@@ -192,7 +192,7 @@ def class_pairs(refs, qrys, costs, kind):
     assert batch, status.value
     try:
         assert handle.dinara_batch_run(batch) == 0
-        return [int(handle.dinara_batch_stat(batch, 400 + cls)) for cls in range(16)]
+        return [int(handle.dinara_batch_stat(batch, 400 + cls)) for cls in range(22)]
     finally:
         handle.dinara_batch_free(batch)
 
@@ -210,18 +210,18 @@ def test_block_bands_one_and_several_waves():
     qrys += ["ACGT" * 40, "GT" * 200, "TTAC" * 300]
     for costs in (da.Costs.affine(4, 6, 2), da.Costs.edit()):
         swept = class_pairs(refs, qrys, costs, 1)
-        assert sum(swept[11:16]) > 0 and sum(swept[12:16]) > 0, swept  # several-wave blocks ran
+        assert sum(swept[10:22]) > 0 and sum(swept[12:22]) > 0, swept  # blocks of several waves ran
         check_alignments(refs, qrys, costs)
         check_distances(refs, qrys, costs)
 
 
 def test_widest_band_distances():
-    # First bands past 8192 diagonals: the 16384-diagonal block, distances only.
+    # First bands past 8192 diagonals: the 12288- and 16384-diagonal blocks, distances only.
     rng = random.Random(41)
     refs = ["".join(rng.choice("ACGT") for _ in range(n)) for n in (11000, 9500)]
     qrys = [refs[0][:1500], mutated(refs[1][2000:3200], 0.1, rng)]
     swept = class_pairs(refs, qrys, da.Costs.affine(4, 6, 2), 0)
-    assert swept[15] > 0, swept
+    assert swept[20] + swept[21] > 0, swept
     check_distances(refs, qrys, da.Costs.affine(4, 6, 2))
     check_distances(refs, qrys, da.Costs.edit())
 

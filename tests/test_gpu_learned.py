@@ -161,3 +161,28 @@ def test_training_seeds_prove_more_pairs_and_change_nothing():
         assert seeded_first > plain_first + 0.3 * len(eval_r), (plain_first, seeded_first)
     finally:
         net.free()
+
+
+def test_doubling_policy_changes_no_result():
+    # Plain band doubling (the baseline the learned band is measured against) proves the same costs and CIGARs.
+    refs, qrys = synthetic.uneven(300, seed=23, clip=(50, 2500))
+    h = _lib.lib()
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    costs = _lib.CCosts(4, 6, 2, -1, 0, 0, 0, -1, 0)
+    results = []
+    for doubling in (0, 1):
+        status = ctypes.c_int64(0)
+        b = h.dinara_batch_new(0, 1, len(refs), *R.args(), *Q.args(), ctypes.byref(costs), None, None, 4,
+                               ctypes.byref(status))
+        assert b, status.value
+        assert h.dinara_batch_set_retry(b, doubling) == 0
+        assert h.dinara_batch_run(b) == 0
+        found = (_lib.CAlignment * len(refs))()
+        statuses = (ctypes.c_int64 * len(refs))()
+        assert h.dinara_batch_alignments(b, found, statuses) == 0
+        results.append([(statuses[i], found[i].cost, ctypes.string_at(found[i].cigar, found[i].cigar_length))
+                        for i in range(len(refs))])
+        for i in range(len(refs)):
+            h.dinara_free(found[i].cigar)
+        h.dinara_batch_free(b)
+    assert results[0] == results[1]
