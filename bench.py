@@ -45,6 +45,10 @@ CLASS_NAMES = ["sweep_group<1>", "sweep_group<2>", "sweep_group<4>", "sweep_grou
 CLASS_WIDTHS = [32, 64, 128, 256, 64, 128, 256, 512, 1024, 2048, 512, 1024, 1536, 2048, 3072, 4096, 5120, 6144, 7168,
                 8192, 12288, 16384]
 CLASSES = len(CLASS_NAMES)
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this bench's
+# default affine step (100,000 pairs of 250 bp), each scaled by what the same counter reads for a plain 1 GiB
+# device copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 (profiles/sweep_group_affine_100k_x_250_hbm_counters.md).
+MEASURED_TRAFFIC = {("affine", 100_000, "sweep_group<2>"): {"read": 119.1e6, "written": 823.3e6}}
 
 
 def host_cores() -> int:
@@ -580,6 +584,13 @@ def main() -> int:
     width = CLASS_WIDTHS[dominant]
     swept_cells = float(np.mean(n)) * width * class_pairs[dominant] / launches
     swept_ops = swept_cells * OPS_PER_CELL / (launch_ms / 1e3) / 1e12
+    measured = MEASURED_TRAFFIC.get(("affine", pairs, CLASS_NAMES[dominant]))
+    traffic = None
+    if measured is not None:
+        moved = measured["read"] + measured["written"]
+        traffic = {"bytes": moved, "read": measured["read"], "written": measured["written"],
+                   "hbm_gbs": round(moved / (launch_ms / 1e3) / 1e9, 1),
+                   "hbm_frac": round(moved / (launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
     roofline = {
         "bound": "valu",
         "kernel": CLASS_NAMES[dominant],
@@ -587,7 +598,7 @@ def main() -> int:
         "peak": VALU_PEAK_TOPS,
         "unit": "TOP/s",
         "frac": round(achieved / VALU_PEAK_TOPS, 4),
-        "traffic": None,
+        "traffic": traffic,
         "launch_ms": round(launch_ms, 4),
         "swept_cells_rate_tops": round(swept_ops, 3),
         "note": "int32 VALU peak (no int MFMA in this path); achieved counts proven-band cells x 12 ops",
