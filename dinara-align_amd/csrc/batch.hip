@@ -17,6 +17,7 @@
 #include "../../include/dinara.h"
 #include "common.h"
 #include "device.h"
+#include "edit.h"
 #include "host.h"
 #include "learned.h"
 
@@ -179,6 +180,7 @@ struct dinara_batch {
         cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, walk_flags;
     ~dinara_batch() {
         for (DeviceBuffer *buffer : arenas) delete buffer;
+        if (edit_graph) (void)hipGraphExecDestroy(edit_graph);
     }
     // the last run's results, by slot
     std::vector<int32_t> cost_h, status_h, cigar_length_h;
@@ -188,9 +190,28 @@ struct dinara_batch {
     // statistics of the last run
     double run_ms = 0, sweep_ms = 0, walk_ms = 0, launches = 0, first_proven = 0;
     double class_ms[CLASSES] = {}, class_launches[CLASSES] = {}, class_pairs[CLASSES] = {};
+
+    // The unit-cost path (edit.hip): unit costs times `unit_scale` and no user band.
+    bool unit = false;
+    int unit_scale = 1;
+    bool use_graph = true;                // replay the run's launches as one captured hipGraph
+    bool persistent = true;               // pair groups refill from their queue; else one pair a group
+    int grid_per_cu = 8;                  // persistent workgroups a compute unit
+    int start_bin[EDIT_LENGTH_CLASSES] = {};  // the least first bin, by the pair's longer side
+    std::vector<int32_t> edit_first_low;  // per slot: its first band's lowest diagonal
+    std::vector<int32_t> edit_first_bin;
+    DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
+        trace_counts, trace_offset, walk_list, walk_count, deferred, deferred_count, deferred_held, edit_arena,
+        arena_used, edit_stats;
+    size_t arena_capacity = 0;
+    hipGraphExec_t edit_graph = nullptr;
+    double edit_ms[EDIT_BINS] = {}, trace_ms[EDIT_BINS] = {}, edit_sweeps[EDIT_BINS] = {},
+           trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0;
 };
 
 namespace {
+
+int64_t unit_lists(dinara_batch *b);
 
 // Builds the tape: pairs by length, longest first, 64 to a group, each group's words transposed.
 int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qrys) {
@@ -388,6 +409,8 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4) ||
         !b->walk_flags.reserve(std::max<int64_t>(slots, 1) * 8))
         return DINARA_OUT_OF_MEMORY;
+    if (b->unit)
+        if (const int64_t code = unit_lists(b)) return code;
     if (b->kind == DINARA_BATCH_ALIGNMENTS) {
         if (!b->cigar_scratch.reserve(std::max<int64_t>(scratch, 1)) ||
             !b->cigar_scratch_base.upload(ctx, b->scratch_base.data(), b->scratch_base.size() * 8) ||
@@ -484,11 +507,224 @@ int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t coun
     return 0;
 }
 
+// ------------------------------------------------------------------ the unit-cost path
+
+// The unit path's first queues: each pair in the bin of its first band, no narrower than its length class's start
+// bin, placed around the start and end diagonals. Caller holds the device lock.
+int64_t unit_lists(dinara_batch *b) {
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    const size_t cap = (size_t)std::max<int64_t>(slots, 1);
+    std::vector<int32_t> lists(EDIT_BINS * cap, 0), counts(EDIT_BINS, 0);
+    b->edit_first_low.assign(cap, 0);
+    b->edit_first_bin.assign(cap, -1);
+    for (int64_t s = 0; s < slots; ++s) {
+        const int n = b->rows[s], m = b->cols[s];
+        const int lo = b->first_low[s], hi = b->first_high[s];
+        const int bin = edit_bin_for(hi - lo + 1, b->start_bin[edit_length_class(std::max(n, m))]);
+        if (bin < 0) continue;  // wider than the widest band: left unsettled
+        b->edit_first_low[s] = edit_place(lo, hi, edit_width(bin), n, m);
+        b->edit_first_bin[s] = bin;
+        lists[bin * cap + counts[bin]++] = (int32_t)s;
+    }
+    if (b->edit_graph) {
+        (void)hipGraphExecDestroy(b->edit_graph);
+        b->edit_graph = nullptr;
+    }
+    DeviceContext *ctx = b->ctx;
+    if (!b->edit_first_lists.upload(ctx, lists.data(), lists.size() * 4) ||
+        !b->edit_first_counts.upload(ctx, counts.data(), counts.size() * 4) ||
+        !b->edit_first_low_d.upload(ctx, b->edit_first_low.data(), cap * 4) || !b->edit_band_low.reserve(cap * 4) ||
+        !b->edit_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_counts.reserve(2 * EDIT_BINS * 4) ||
+        !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
+        !b->trace_offset.reserve(cap * 8) || !b->walk_list.reserve(cap * 4) || !b->walk_count.reserve(4) ||
+        !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
+        !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 1) * 4))
+        return DINARA_OUT_OF_MEMORY;
+    return 0;
+}
+
+EditArgs edit_args(dinara_batch *b) {
+    EditArgs e{};
+    e.s = sweep_args(b);
+    // The certificate and the cap in unit terms: every cost is `unit_scale` times the unit cost.
+    e.s.costs = CostParams{1, 0, 1, 0, 1, 1, 1};
+    e.s.cap = b->options.max_cost >= 0 ? b->options.max_cost / b->unit_scale : INT64_MAX / 4;
+    e.scale = b->unit_scale;
+    e.capacity = (int)std::max<size_t>(b->slot_pair.size(), 1);
+    e.band_low = static_cast<int32_t *>(b->edit_band_low.ptr);
+    e.first_band_low = static_cast<const int32_t *>(b->edit_first_low_d.ptr);
+    e.lists = static_cast<int32_t *>(b->edit_lists.ptr);
+    e.counts = static_cast<int32_t *>(b->edit_counts.ptr);
+    e.first_lists = static_cast<int32_t *>(b->edit_first_lists.ptr);
+    e.first_counts = static_cast<const int32_t *>(b->edit_first_counts.ptr);
+    e.trace_lists = static_cast<int32_t *>(b->trace_lists.ptr);
+    e.trace_counts = static_cast<int32_t *>(b->trace_counts.ptr);
+    e.trace_offset = static_cast<int64_t *>(b->trace_offset.ptr);
+    e.walk_list = static_cast<int32_t *>(b->walk_list.ptr);
+    e.walk_count = static_cast<int32_t *>(b->walk_count.ptr);
+    e.deferred = static_cast<int32_t *>(b->deferred.ptr);
+    e.deferred_count = static_cast<int32_t *>(b->deferred_count.ptr);
+    e.arena = static_cast<uint8_t *>(b->edit_arena.ptr);
+    e.arena_used = static_cast<unsigned long long *>(b->arena_used.ptr);
+    e.arena_capacity = b->arena_capacity;
+    e.stats = static_cast<uint32_t *>(b->edit_stats.ptr);
+    e.one_pair = b->persistent ? 0 : 1;
+    return e;
+}
+
+// Workgroups of bin `bin`'s sweep: enough for every pair at once, at most `grid_per_cu` a compute unit when the
+// groups refill from the queue.
+int edit_grid(const dinara_batch *b, int bin) {
+    const int64_t every = ((int64_t)b->slot_pair.size() * edit_lanes(bin) + 255) / 256;
+    if (!b->persistent) return (int)std::max<int64_t>(1, every);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(every, (int64_t)b->ctx->compute_units * b->grid_per_cu));
+}
+
+// The arena the traced bands' decisions take: the batch's bound (no traced band is wider than its whole matrix),
+// at most 32 GiB or half the free memory, at least 1 MiB. DINARA_TRACE_ARENA sets it in bytes.
+size_t arena_bytes(const dinara_batch *b) {
+    if (const char *env = std::getenv("DINARA_TRACE_ARENA")) return std::max<size_t>(std::strtoull(env, nullptr, 10), 4096);
+    size_t bound = 0;
+    for (size_t s = 0; s < b->slot_pair.size(); ++s) {
+        const int n = b->rows[s], m = b->cols[s];
+        const int bin = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
+        bound += (size_t)m * edit_lanes(bin) * edit_words(bin) * 16;
+    }
+    size_t free_bytes = 0, total = 0;
+    (void)hipMemGetInfo(&free_bytes, &total);
+    const size_t most = std::min<size_t>(size_t(32) << 30, free_bytes / 2);
+    return std::max<size_t>(std::min(bound, most), size_t(1) << 20);
+}
+
+// The run's launches in order: reset, the sweep of every bin, then (alignments) the tracing sweep of every bin and
+// the walk. `timed` records an event pair around each launch (not under capture).
+int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace) {
+    DeviceContext *ctx = b->ctx;
+    const int slots = (int)b->slot_pair.size();
+    launch_edit_reset(a, slots, ctx->stream);
+    for (int bin = 0; bin < EDIT_BINS; ++bin) {
+        if (timed) (void)hipEventRecord(ctx->event(2 * bin), ctx->stream);
+        launch_edit_sweep(a, bin, false, edit_grid(b, bin), ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(2 * bin + 1), ctx->stream);
+    }
+    if (trace) {
+        for (int bin = 0; bin < EDIT_BINS; ++bin) {
+            if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin)), ctx->stream);
+            launch_edit_sweep(a, bin, true, edit_grid(b, bin), ctx->stream);
+            if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin) + 1), ctx->stream);
+        }
+        if (timed) (void)hipEventRecord(ctx->event(4 * EDIT_BINS), ctx->stream);
+        launch_edit_walk(a, slots, ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(4 * EDIT_BINS + 1), ctx->stream);
+    }
+    return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
+}
+
+// The unit-cost run: one captured graph (or the same launches one by one, timed), then further tracing passes for
+// pairs the arena had no room for. `trace` false skips the tracing (the autotuner's runs).
+int64_t run_unit(dinara_batch *b, bool trace) {
+    DeviceContext *ctx = b->ctx;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    trace = trace && b->kind == DINARA_BATCH_ALIGNMENTS;
+    b->launches = b->first_proven = b->graph_used = b->trace_passes = 0;
+    b->run_ms = b->sweep_ms = b->walk_ms = 0;
+    for (int bin = 0; bin < EDIT_BINS; ++bin) b->edit_ms[bin] = b->trace_ms[bin] = b->edit_sweeps[bin] = b->trace_sweeps[bin] = 0;
+    if (slots == 0) {
+        b->ran = true;
+        return 0;
+    }
+    if (b->kind == DINARA_BATCH_ALIGNMENTS && !b->edit_arena.ptr) {
+        const size_t bytes = arena_bytes(b);
+        if (!b->edit_arena.reserve(bytes)) return DINARA_OUT_OF_MEMORY;
+        b->arena_capacity = bytes;
+        if (b->edit_graph) {
+            (void)hipGraphExecDestroy(b->edit_graph);
+            b->edit_graph = nullptr;
+        }
+    }
+    const EditArgs a = edit_args(b);
+    const bool graph = b->use_graph && trace == (b->kind == DINARA_BATCH_ALIGNMENTS);
+    if (graph) {
+        if (!b->edit_graph) {
+            hipGraph_t captured = nullptr;
+            if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return DINARA_OUT_OF_MEMORY;
+            const int64_t code = enqueue_unit(b, a, false, trace);
+            const hipError_t ended = hipStreamEndCapture(ctx->stream, &captured);
+            if (code || ended != hipSuccess) return DINARA_OUT_OF_MEMORY;
+            const hipError_t made = hipGraphInstantiate(&b->edit_graph, captured, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(captured);
+            if (made != hipSuccess) {
+                b->edit_graph = nullptr;
+                return DINARA_OUT_OF_MEMORY;
+            }
+        }
+        if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess ||
+            hipGraphLaunch(b->edit_graph, ctx->stream) != hipSuccess)
+            return DINARA_OUT_OF_MEMORY;
+        b->graph_used = 1;
+    } else {
+        if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        if (const int64_t code = enqueue_unit(b, a, true, trace)) return code;
+    }
+    if (hipEventRecord(ctx->sweep_stop, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    // Pairs whose decisions found no room: trace them in further passes, the arena emptied each time.
+    int32_t deferred = 0;
+    if (trace) {
+        if (!b->deferred_count.read(ctx, &deferred, 4)) return DINARA_OUT_OF_MEMORY;
+        while (deferred > 0) {
+            b->trace_passes += 1;
+            if (hipMemcpyAsync(b->deferred_held.ptr, b->deferred.ptr, (size_t)deferred * 4, hipMemcpyDeviceToDevice,
+                               ctx->stream) != hipSuccess)
+                return DINARA_OUT_OF_MEMORY;
+            launch_edit_requeue(a, static_cast<const int32_t *>(b->deferred_held.ptr), deferred, ctx->stream);
+            for (int bin = 0; bin < EDIT_BINS; ++bin) launch_edit_sweep(a, bin, true, edit_grid(b, bin), ctx->stream);
+            launch_edit_walk(a, (int)slots, ctx->stream);
+            int32_t left = 0;
+            if (hipPeekAtLastError() != hipSuccess || !b->deferred_count.read(ctx, &left, 4)) return DINARA_OUT_OF_MEMORY;
+            if (left >= deferred) return DINARA_OUT_OF_MEMORY;  // a pair larger than the whole arena
+            deferred = left;
+        }
+    }
+    if (hipEventRecord(ctx->stop, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->stop) != hipSuccess)
+        return DINARA_OUT_OF_MEMORY;
+    float total = 0, sweeps = 0;
+    (void)hipEventElapsedTime(&total, ctx->start, ctx->stop);
+    (void)hipEventElapsedTime(&sweeps, ctx->start, ctx->sweep_stop);
+    b->run_ms = total;
+    b->sweep_ms = sweeps;
+    uint32_t stats[2 * EDIT_BINS + 1] = {};
+    if (!b->edit_stats.read(ctx, stats, sizeof(stats))) return DINARA_OUT_OF_MEMORY;
+    for (int bin = 0; bin < EDIT_BINS; ++bin) {
+        b->edit_sweeps[bin] = stats[bin];
+        b->trace_sweeps[bin] = stats[EDIT_BINS + bin];
+        if (!graph) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ctx->event(2 * bin), ctx->event(2 * bin + 1));
+            b->edit_ms[bin] = ms;
+            if (trace) {
+                (void)hipEventElapsedTime(&ms, ctx->event(2 * (EDIT_BINS + bin)), ctx->event(2 * (EDIT_BINS + bin) + 1));
+                b->trace_ms[bin] = ms;
+            }
+        }
+    }
+    if (!graph && trace) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->event(4 * EDIT_BINS), ctx->event(4 * EDIT_BINS + 1));
+        b->walk_ms = ms;
+    }
+    b->first_proven = (double)(slots - stats[2 * EDIT_BINS]);
+    b->launches = graph ? 1 : 1 + EDIT_BINS * (trace ? 2 : 1) + (trace ? 1 : 0);
+    b->ran = true;
+    return 0;
+}
+
 int64_t run(dinara_batch *b) {
     DeviceContext *ctx = b->ctx;
     std::lock_guard<std::mutex> serial(ctx->lock);
     DeviceGuard guard(ctx);
     (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
+    if (b->unit) return run_unit(b, true);
     const int64_t slots = (int64_t)b->slot_pair.size();
     b->launches = b->first_proven = 0;
     b->run_ms = b->sweep_ms = b->walk_ms = 0;
@@ -640,6 +876,12 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     const int64_t clamp = int64_t(1) << 30;
     b->user_low = (int)std::max<int64_t>(-o.band_high, -clamp);
     b->user_high = (int)std::min<int64_t>(-o.band_low, clamp);
+    // Unit costs times a scale, with no user band, take the bit-parallel path (DINARA_GOTOH_UNIT=1 keeps them on the
+    // gap-cost sweeps, for comparison).
+    b->unit = !b->user_band && parsed.opening == 0 && parsed.del_opening == 0 &&
+              parsed.extension == parsed.mismatch && parsed.del_extension == parsed.mismatch &&
+              !(std::getenv("DINARA_GOTOH_UNIT") && std::atoi(std::getenv("DINARA_GOTOH_UNIT")) != 0);
+    b->unit_scale = (int)parsed.mismatch;
     const int64_t dearest = std::max<int64_t>(
         parsed.mismatch, std::max(parsed.opening + parsed.extension, parsed.del_opening + parsed.del_extension));
     b->dearest = std::max<int64_t>(dearest, 1);
@@ -851,7 +1093,87 @@ int64_t seed_slots(dinara_batch *b, GuessOf guess_of) {
         !b->first_high_d.write(ctx, b->first_high.data(), slots * 4) ||
         !b->first_lists_d.upload(ctx, first_all.data(), std::max<size_t>(first_all.size(), 1) * 4))
         return DINARA_OUT_OF_MEMORY;
+    if (b->unit) return unit_lists(b);
     return 0;
+}
+
+void drop_graph(dinara_batch *b) {
+    if (b->edit_graph) {
+        DeviceGuard guard(b->ctx);
+        (void)hipGraphExecDestroy(b->edit_graph);
+        b->edit_graph = nullptr;
+    }
+}
+
+// Tunes the unit-cost sweep's launch shape on the batch itself: first the workgroups a compute unit, then each
+// length class's least first bin (heaviest class first, widening while the first pass gets faster). Each
+// candidate is timed as the faster of two first passes (no tracing).
+int64_t autotune(dinara_batch *b, int64_t *chosen) {
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    auto report = [&]() {
+        if (!chosen) return;
+        chosen[0] = b->grid_per_cu;
+        for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) chosen[1 + c] = b->start_bin[c];
+    };
+    if (!b->unit || slots == 0) {
+        report();
+        return 0;
+    }
+    DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
+    DeviceGuard guard(ctx);
+    (void)hipGetLastError();
+    int64_t failure = 0;
+    auto measure = [&]() {
+        double best = 1e300;
+        for (int i = 0; i < 2; ++i) {
+            if (const int64_t code = run_unit(b, false)) {
+                failure = code;
+                return 1e300;
+            }
+            best = std::min(best, b->run_ms);
+        }
+        return best;
+    };
+    double best = measure();
+    int best_grid = b->grid_per_cu;
+    for (int grid : {2, 4, 8, 16}) {
+        if (grid == best_grid) continue;
+        b->grid_per_cu = grid;
+        drop_graph(b);
+        const double t = measure();
+        if (t < best) {
+            best = t;
+            best_grid = grid;
+        }
+    }
+    b->grid_per_cu = best_grid;
+    drop_graph(b);
+    // Length classes by weight (the sum of their pairs' squared longer sides), heaviest first.
+    double weight[EDIT_LENGTH_CLASSES] = {};
+    for (int64_t s = 0; s < slots; ++s) {
+        const double longer = std::max(b->rows[s], b->cols[s]);
+        weight[edit_length_class((int)longer)] += longer * longer;
+    }
+    int order[EDIT_LENGTH_CLASSES];
+    std::iota(order, order + EDIT_LENGTH_CLASSES, 0);
+    std::sort(order, order + EDIT_LENGTH_CLASSES, [&](int x, int y) { return weight[x] > weight[y]; });
+    for (int c : order) {
+        if (weight[c] == 0) continue;
+        int best_start = b->start_bin[c];
+        for (int start = best_start + 1; start < EDIT_BINS; ++start) {
+            b->start_bin[c] = start;
+            if (const int64_t code = unit_lists(b)) return code;
+            const double t = measure();
+            if (t >= best) break;
+            best = t;
+            best_start = start;
+        }
+        b->start_bin[c] = best_start;
+        if (const int64_t code = unit_lists(b)) return code;
+    }
+    report();
+    return failure;
 }
 
 }  // namespace
@@ -890,7 +1212,41 @@ int64_t dinara_batch_seed_learned(dinara_batch *b, dinara_mlp *net, double margi
 int64_t dinara_batch_set_retry(dinara_batch *b, int64_t doubling) {
     if (!b) return DINARA_INVALID_MODE;
     b->doubling = doubling != 0;
+    drop_graph(b);
     return 0;
+}
+
+int64_t dinara_batch_set_graph(dinara_batch *b, int64_t on) {
+    if (!b) return DINARA_INVALID_MODE;
+    b->use_graph = on != 0;
+    return 0;
+}
+
+int64_t dinara_batch_set_persistent(dinara_batch *b, int64_t on) {
+    if (!b) return DINARA_INVALID_MODE;
+    b->persistent = on != 0;
+    drop_graph(b);
+    return 0;
+}
+
+int64_t dinara_batch_set_tuning(dinara_batch *b, int64_t grid_per_cu, const int64_t *start_bins) {
+    if (!b || grid_per_cu < 1 || grid_per_cu > 64) return DINARA_INVALID_MODE;
+    if (start_bins)
+        for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c)
+            if (start_bins[c] < 0 || start_bins[c] >= EDIT_BINS) return DINARA_INVALID_MODE;
+    b->grid_per_cu = (int)grid_per_cu;
+    if (start_bins)
+        for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) b->start_bin[c] = (int)start_bins[c];
+    drop_graph(b);
+    if (!b->unit || b->slot_pair.empty()) return 0;
+    std::lock_guard<std::mutex> serial(b->ctx->lock);
+    DeviceGuard guard(b->ctx);
+    return unit_lists(b);
+}
+
+int64_t dinara_batch_autotune(dinara_batch *b, int64_t *chosen) {
+    if (!b) return DINARA_INVALID_MODE;
+    return autotune(b, chosen);
 }
 
 void dinara_batch_free(dinara_batch *batch) {
@@ -909,7 +1265,19 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 2: return b->launches;
         case 4: return b->first_proven;
         case 5: return b->walk_ms;
+        case 6: return b->unit ? 1 : 0;
+        case 7: return b->graph_used;
+        case 8: return b->trace_passes;
         default: break;
+    }
+    if (what >= 600 && what < 1000 && what % 100 < EDIT_BINS) {
+        const int bin = (int)(what % 100);
+        switch (what / 100) {
+            case 6: return b->edit_ms[bin];
+            case 7: return b->edit_sweeps[bin];
+            case 8: return b->trace_ms[bin];
+            case 9: return b->trace_sweeps[bin];
+        }
     }
     if (what >= 100 && what < 500 && what % 100 < CLASSES) {
         const int c = (int)(what % 100);

@@ -1,0 +1,88 @@
+// The unit-cost path: global edit distance and its CIGAR by a banded bit-parallel sweep (edit.hip).
+//
+// A band of W = 64 x lanes x words diagonals is a bit-vector window over one column's rows that moves down one row
+// a column, so bit i always stands for diagonal high - i. Each column takes one step of the bit-vector recurrence for
+// edit distance (Myers 1999, with Hyyro's banded form): Pv/Mv hold the column's vertical differences (+1/-1),
+// Ph/Mh the row's horizontal ones. The row above the window and the cell left of the window's new bottom row lie
+// outside the band; the sweep stands in a value one more than the in-band neighbour they share (a +1 carry into the
+// top, a +1 vertical difference under the bottom), which the diagonal move always matches or beats, so every in-band
+// value is the least cost of a path inside the band. Rows above the matrix are the boundary's continuation (cost
+// column - row) and rows below it feed nothing, so their letters may be anything.
+//
+// Pairs are binned by band width. A wave holds 64 / lanes pairs, each on `lanes` adjacent lanes, and every pair
+// group takes its next pair from its bin's queue the moment it finishes the last one (a persistent launch): pairs of
+// very different lengths share waves without waiting for each other. A pair its band does not prove is filed into a
+// wider bin, which runs later in the same sequence of launches; that sequence has a fixed shape, so one hipGraph
+// replays it.
+#pragma once
+
+#include "common.h"
+
+namespace dinara {
+
+// Bins: 1, 2, 4, ..., 64 lanes of one 64-bit word each (64 to 4096 diagonals), then 64 lanes of 2 and 4 words.
+constexpr int EDIT_BINS = 9;
+__host__ __device__ constexpr int edit_lanes(int bin) { return bin < 7 ? 1 << bin : 64; }
+__host__ __device__ constexpr int edit_words(int bin) { return bin < 7 ? 1 : (bin == 7 ? 2 : 4); }
+__host__ __device__ constexpr int edit_width(int bin) { return 64 * edit_lanes(bin) * edit_words(bin); }
+
+struct EditArgs {
+    SweepArgs s;            // tape, results, certificate policy (unit costs), CIGAR output
+    int scale;              // the batch's costs are `scale` times unit costs
+    int capacity;           // slots a list holds
+    int32_t *band_low;      // per slot: the lowest diagonal of the band its next sweep takes
+    const int32_t *first_band_low;  // per slot: its first band's, copied into `band_low` at the start of a run
+    int32_t *lists;         // EDIT_BINS sweep queues of `capacity` slots
+    int32_t *counts;        // [0, EDIT_BINS) queue lengths, [EDIT_BINS, 2 EDIT_BINS) queue heads
+    int32_t *first_lists;   // the first sweep's queues, copied into `lists` at the start of a run
+    const int32_t *first_counts;
+    // alignments: the band each settled pair is traced in, its decisions' place in the arena, and the queues of the
+    // tracing sweep and of the walk
+    int32_t *trace_lists;   // EDIT_BINS queues of `capacity` slots
+    int32_t *trace_counts;  // lengths, then heads
+    int64_t *trace_offset;  // per slot: byte offset of its decisions in the arena
+    int32_t *walk_list;
+    int32_t *walk_count;
+    int32_t *deferred;      // settled pairs whose decisions found no room in the arena this pass
+    int32_t *deferred_count;
+    uint8_t *arena;
+    unsigned long long *arena_used;
+    unsigned long long arena_capacity;
+    uint32_t *stats;        // sweeps begun by bin, then trace sweeps by bin, then pairs their first band left unproven
+    int one_pair;           // 1: each pair group sweeps one pair and stops (no refill), for comparison
+};
+
+// Pairs' longer sides split at 256, 1024, 4096 and 16384 letters: each class has its own least first bin, the
+// tuned start of its band doubling.
+constexpr int EDIT_LENGTH_CLASSES = 5;
+__host__ __device__ inline int edit_length_class(int longer) {
+    return longer < 256 ? 0 : longer < 1024 ? 1 : longer < 4096 ? 2 : longer < 16384 ? 3 : 4;
+}
+
+// The bin whose band holds `need` diagonals, at least `from`; -1 when none does.
+__host__ __device__ inline int edit_bin_for(int need, int from) {
+    for (int bin = from; bin < EDIT_BINS; ++bin)
+        if (need <= edit_width(bin)) return bin;
+    return -1;
+}
+
+// Where a band of `width` diagonals holding [want_low, want_high] starts: centred, kept inside the matrix's
+// diagonals (-rows - 1 .. cols + 1) where it can be.
+__host__ __device__ inline int edit_place(int want_low, int want_high, int width, int rows, int cols) {
+    int low = want_low - (width - (want_high - want_low + 1)) / 2;
+    if (low + width - 1 > cols + 1) low = cols + 2 - width;
+    if (low < -rows - 1) low = -rows - 1;
+    if (low > want_low) low = want_low;
+    if (low + width - 1 < want_high) low = want_high - width + 1;
+    return low;
+}
+
+// Host-side entry points of edit.hip. `grid` is the workgroups of each persistent launch (256 threads each).
+void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream);
+void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream);
+void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream);
+// Empties the arena and the trace and walk queues, then queues `count` deferred pairs (`held`, a copy of the
+// deferred list) for a further tracing pass.
+void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream);
+
+}  // namespace dinara

@@ -1,0 +1,502 @@
+// The unit-cost path's kernels: the persistent banded bit-parallel sweep, the walk over its 2-bit decisions, and
+// the small kernels that reset and requeue its lists (the scheme is described in edit.h).
+//
+// Reference: the unit-cost distance is the reference's lanes path at costs (1, 0, 1) (lanes.mojo:950-1004) and its
+// device kernel's bit-parallel recurrence (device_edit.mojo:29-77); the CIGAR follows the reference's bit-parallel
+// traceback rule (traceback.mojo:400-575): from the corner, a substitution when one is optimal, else a reference
+// letter alone, else a query letter alone, and two equal letters only when no edit is; Ties.LEFT walks both
+// sequences reversed (edit_distance.mojo:295-297). At unit costs that rule and the reference's DP-flag rule
+// (lanes.mojo:905-923) give the same CIGAR.
+#include "bands.h"
+#include "edit.h"
+
+namespace dinara {
+namespace {
+
+// Lane t receives lane t - 1's value (FROM_LOWER) or lane t + 1's; the wave's end lane keeps its own.
+template <bool FROM_LOWER>
+__device__ __forceinline__ uint32_t wave_neighbour(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, FROM_LOWER ? 0x138 : 0x130, 0xF, 0xF, false);
+}
+
+// The even bits of a 32-bit word packed into its low 16.
+__device__ __forceinline__ uint32_t even_bits(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+
+// Bit b of byte q of a 32-bit word, for q = 0..3, packed into its low 4 bits.
+__device__ __forceinline__ uint32_t byte_bits(uint32_t x, int b) {
+    x = (x >> b) & 0x01010101u;
+    return (x | (x >> 7) | (x >> 14) | (x >> 21)) & 0xFu;
+}
+
+__device__ __forceinline__ void place(uint64_t &plane, uint64_t bits, int shift) {
+    if (shift >= 64 || shift <= -64) return;
+    plane |= shift >= 0 ? bits << shift : bits >> -shift;
+}
+
+// The code planes of 64 letters of one sequence from letter `first` on: plane b bit i is bit b of letter
+// first + i's code, 0 for letters outside 0..n-1. `text` holds word w of the sequence at text[64 w].
+template <int BITS>
+__device__ __forceinline__ void load_planes(const uint32_t *text, int n, int first, uint64_t (&planes)[BITS]) {
+    constexpr int PER = 32 / BITS;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) planes[b] = 0;
+    const int w0 = first >= 0 ? first / PER : -((PER - 1 - first) / PER);
+    const int w1 = (first + 63) >= 0 ? (first + 63) / PER : -1;
+    for (int w = w0 < 0 ? 0 : w0; w <= w1 && w * PER < n; ++w) {
+        const uint32_t word = text[(size_t)w * 64];
+        const int valid = n - w * PER < PER ? n - w * PER : PER;
+        const uint64_t keep = (uint64_t(1) << valid) - 1;
+        const int shift = w * PER - first;
+        if constexpr (BITS == 2) {
+            place(planes[0], even_bits(word) & keep, shift);
+            place(planes[1], even_bits(word >> 1) & keep, shift);
+        } else {
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) place(planes[b], byte_bits(word, b) & keep, shift);
+        }
+    }
+}
+
+// One sequence read a letter at a time: the code of letter `at` (0 outside 0..n-1), then the next.
+template <int BITS>
+struct Cursor {
+    const uint32_t *text;
+    int n, at;
+    uint32_t word;
+
+    __device__ void begin(const uint32_t *t, int length, int from) {
+        constexpr int PER = 32 / BITS;
+        text = t;
+        n = length;
+        at = from;
+        word = (at >= 0 && at < n) ? text[(size_t)(at / PER) * 64] : 0u;
+    }
+    __device__ uint32_t next() {
+        constexpr int PER = 32 / BITS;
+        uint32_t code = 0;
+        if (at >= 0 && at < n) {
+            const int within = at % PER;
+            if (within == 0) word = text[(size_t)(at / PER) * 64];
+            code = (word >> (BITS * within)) & ((BITS == 8) ? 0xFFu : 3u);
+        }
+        ++at;
+        return code;
+    }
+};
+
+// ------------------------------------------------------------------ settling and filing (first lanes only)
+
+__device__ void settle(const EditArgs &a, int slot, int n, int m, int64_t found) {
+    const SweepArgs &s = a.s;
+    s.cost[slot] = (int32_t)(found * a.scale);
+    s.status[slot] = 0;
+    if (!s.record || found > s.cap) return;  // distances, or above the cap: no CIGAR asked
+    if (n == 0 || m == 0) {                  // all gaps: nothing to trace
+        s.walk_low[slot] = 0;
+        s.walk_class[slot] = 0;
+        a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+        return;
+    }
+    // The narrowest band the strict certificate proves at the found cost: it holds every optimal path.
+    const int end = m - n;
+    const int64_t target = found + 1;
+    int h = end > 0 ? end : 0, l = end < 0 ? end : 0;
+    while (h < m && off_band(s.costs, h + 1, end) < target) ++h;
+    while (l > -n && off_band(s.costs, l - 1, end) < target) --l;
+    const int bin = edit_bin_for(h - l + 1, 0);
+    const int low = edit_place(l, h, edit_width(bin), n, m);
+    s.walk_low[slot] = low;
+    s.walk_class[slot] = bin;
+    const unsigned long long bytes = (unsigned long long)m * edit_lanes(bin) * edit_words(bin) * 16ull;
+    const unsigned long long at = atomicAdd(a.arena_used, bytes);
+    if (at + bytes > a.arena_capacity) {
+        a.deferred[atomicAdd(a.deferred_count, 1)] = slot;
+        return;
+    }
+    a.trace_offset[slot] = (int64_t)at;
+    a.trace_lists[(size_t)bin * a.capacity + atomicAdd(&a.trace_counts[bin], 1)] = slot;
+}
+
+__device__ void refile(const EditArgs &a, int slot, int n, int m, int lo, int hi, int want_low, int want_high,
+                       int bin) {
+    if (a.s.status[slot] < 0) atomicAdd(&a.stats[2 * EDIT_BINS], 1u);  // left unproven by its first band
+    if (a.s.doubling) {
+        // Plain band doubling: the next bin, twice as wide, centred on the band just swept.
+        want_low = lo;
+        want_high = hi;
+    }
+    const int next = a.s.doubling ? (bin + 1 < EDIT_BINS ? bin + 1 : -1)
+                                  : edit_bin_for(want_high - want_low + 1, bin + 1);
+    if (next < 0) {
+        a.s.status[slot] = 2;
+        return;
+    }
+    a.band_low[slot] = edit_place(want_low, want_high, edit_width(next), n, m);
+    a.s.status[slot] = 1;
+    a.lists[(size_t)next * a.capacity + atomicAdd(&a.counts[next], 1)] = slot;
+}
+
+// ------------------------------------------------------------------ the sweep
+
+// Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. TRACE: the second pass of an
+// alignment batch, which sweeps each settled pair's traced band again and keeps a 2-bit decision per cell (0 two
+// unequal letters, 1 a reference letter alone, 2 a query letter alone, 3 two equal letters: the walk's first true
+// choice in that order) in two bit-planes, plane-major per word: word w's column c at arena + offset + (w m + c) 16.
+template <int L, int K, int BITS, bool TRACE>
+__global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
+    constexpr int W = 64 * L * K;
+    const int lane = threadIdx.x & 63;
+    const int k = lane & (L - 1);
+    const bool first_lane = k == 0, last_lane = k == L - 1;
+    const int32_t *queue = (TRACE ? a.trace_lists : a.lists) + (size_t)bin * a.capacity;
+    int32_t *counts = TRACE ? a.trace_counts : a.counts;
+    const int count = counts[bin];
+    int32_t *head = counts + EDIT_BINS + bin;
+
+    int slot = -1;
+    bool drained = false;
+    int n = 0, m = 0, lo = 0, hi = 0, j = 0, top = 0;
+    uint64_t pv[K], mv[K], pl[BITS][K];
+    Cursor<BITS> bottom, query;
+    bottom.begin(nullptr, 0, 0);
+    query.begin(nullptr, 0, 0);
+    uint8_t *trace = nullptr;
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+        pv[kk] = mv[kk] = 0;
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) pl[b][kk] = 0;
+    }
+
+    while (true) {
+        // Pair groups with nothing to sweep claim the next pairs: one atomic a wave.
+        const bool asking = first_lane && slot < 0 && !drained;
+        if (a.one_pair) drained = true;  // a group asks once
+        const unsigned long long askers = __ballot(asking);
+        if (askers) {
+            const int leader = __builtin_ctzll(askers);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(head, __popcll(askers));
+            base = __shfl(base, leader);
+            const int mine = base + __popcll(askers & ((1ull << lane) - 1ull));
+            const int idx = __shfl(mine, lane - k);
+            const bool group_asked = __shfl((int)asking, lane - k) != 0;
+            if (group_asked) {
+                if (idx < count) {
+                    slot = queue[idx];
+                    n = a.s.tape.rows[slot];
+                    m = a.s.tape.cols[slot];
+                    lo = TRACE ? a.s.walk_low[slot] : a.band_low[slot];
+                    hi = lo + W - 1;
+                    j = 0;
+                    top = 0;
+                    const int g = slot >> 6;
+                    const uint32_t *ref = a.s.tape.words + a.s.tape.group_base[g] + (slot & 63);
+                    const uint32_t *qry = ref + (size_t)a.s.tape.group_ref_words[g] * 64;
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) {
+                        const int w = k * K + kk;
+                        // Column 0: cost |row|, so +1 below row 0 and -1 from it up (rows above the matrix).
+                        const int t = hi + 1 - 64 * w;
+                        pv[kk] = t <= 0 ? ~0ull : t >= 64 ? 0ull : ~0ull << t;
+                        mv[kk] = ~pv[kk];
+                        uint64_t planes[BITS];
+                        load_planes<BITS>(ref, n, 64 * w - hi - 1, planes);
+#pragma unroll
+                        for (int b = 0; b < BITS; ++b) pl[b][kk] = planes[b];
+                    }
+                    bottom.begin(ref, n, -lo);
+                    query.begin(qry, m, 0);
+                    if (TRACE) trace = a.arena + a.trace_offset[slot] + (size_t)(k * K) * m * 16;
+                    if (first_lane) atomicAdd(&a.stats[(TRACE ? EDIT_BINS : 0) + bin], 1u);
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__ballot(slot >= 0)) break;
+
+        // Pairs at their last column: the cost at (n, m), then settle, file for a wider band, or queue the walk.
+        const bool finishing = slot >= 0 && j == m;
+        if (__ballot(finishing)) {
+            const int i_n = hi - (m - n);  // the window bit of row n
+            int part = 0;
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                const int w = k * K + kk;
+                const int from = (64 * w > 1 ? 64 * w : 1) - 64 * w;
+                const int to = (i_n < 64 * w + 63 ? i_n : 64 * w + 63) - 64 * w;
+                if (from <= to) {
+                    const uint64_t mask = (to == 63 ? ~0ull : ((1ull << (to + 1)) - 1ull)) & (~0ull << from);
+                    part += __popcll(pv[kk] & mask) - __popcll(mv[kk] & mask);
+                }
+            }
+#pragma unroll
+            for (int d = L / 2; d >= 1; d /= 2) part += __shfl_xor(part, d);
+            const int found = hi + m + top + part;  // `top` is the first lane's
+            if (first_lane && finishing) {
+                if (TRACE) {
+                    // The traced band was proven in the first pass: its cost must agree.
+                    if ((int64_t)found * a.scale != (int64_t)a.s.cost[slot]) a.s.status[slot] = 3;
+                    a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+                } else {
+                    int want_low = 0, want_high = 0;
+                    if (certify(a.s, n, m, found, lo, hi, &want_low, &want_high))
+                        settle(a, slot, n, m, found);
+                    else
+                        refile(a, slot, n, m, lo, hi, want_low, want_high, bin);
+                }
+            }
+            if (finishing) slot = -1;
+            continue;
+        }
+
+        // One column. The window moves down a row: each word takes the bit of the word below; the band's bottom
+        // takes +1 under it (Pv), and the new bottom row's letter.
+        ++j;
+        {
+            uint32_t low = (uint32_t)(pv[0] & 1) | ((uint32_t)(mv[0] & 1) << 1);
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) low |= (uint32_t)(pl[b][0] & 1) << (2 + b);
+            uint32_t in = L > 1 ? wave_neighbour<false>(low) : 0u;
+            if (last_lane) in = 1u | (bottom.next() << 2);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                const bool top_word = kk == K - 1;
+                pv[kk] = (pv[kk] >> 1) | ((top_word ? (uint64_t)(in & 1) : pv[kk + (top_word ? 0 : 1)] & 1) << 63);
+                mv[kk] = (mv[kk] >> 1) | ((top_word ? (uint64_t)((in >> 1) & 1) : mv[kk + (top_word ? 0 : 1)] & 1) << 63);
+#pragma unroll
+                for (int b = 0; b < BITS; ++b)
+                    pl[b][kk] = (pl[b][kk] >> 1) |
+                                ((top_word ? (uint64_t)((in >> (2 + b)) & 1) : pl[b][kk + (top_word ? 0 : 1)] & 1) << 63);
+            }
+        }
+        // The rows whose letter equals the query's letter j - 1.
+        const uint32_t c = query.next();
+        uint64_t eq[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            uint64_t differ = 0;
+#pragma unroll
+            for (int b = 0; b < BITS; ++b) differ |= pl[b][kk] ^ (((c >> b) & 1u) ? ~0ull : 0ull);
+            eq[kk] = ~differ;
+        }
+        // (Eq & Pv) + Pv over the pair's whole window: carries ripple through a lane's words, and across lanes by
+        // generate / propagate ballots (a carry into lane t is bit t of (G + (G | P)) ^ P).
+        uint64_t sum[K];
+        uint32_t carry = 0;
+        bool all_ones = true;
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const uint64_t x = (eq[kk] & pv[kk]) + pv[kk];
+            const uint32_t c1 = x < pv[kk];
+            sum[kk] = x + carry;
+            carry = c1 | (uint32_t)(carry && x == ~0ull);
+            all_ones = all_ones && sum[kk] == ~0ull;
+        }
+        if constexpr (L > 1) {
+            const unsigned long long gen = __ballot(carry && !last_lane);
+            const unsigned long long prop = __ballot(all_ones && !last_lane);
+            const unsigned long long into = (gen + (gen | prop)) ^ prop;
+            if ((into >> lane) & 1ull) {
+                uint32_t up = 1;
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) {
+                    sum[kk] += up;
+                    up = up && sum[kk] == 0;
+                }
+            }
+        }
+        uint64_t ph[K], mh[K], xv[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            xv[kk] = eq[kk] | mv[kk];
+            const uint64_t xh = (sum[kk] ^ pv[kk]) | eq[kk];
+            ph[kk] = mv[kk] | ~(xh | pv[kk]);
+            mh[kk] = pv[kk] & xh;
+        }
+        // Horizontal differences one row down: a +1 carries into the band's top (the stand-in above it).
+        uint32_t carried = (uint32_t)(ph[K - 1] >> 63) | ((uint32_t)(mh[K - 1] >> 63) << 1);
+        carried = L > 1 ? wave_neighbour<true>(carried) : 0u;
+        if (first_lane) carried = 1u;
+        const bool active = slot >= 0;
+#pragma unroll
+        for (int kk = K - 1; kk >= 0; --kk) {
+            const uint64_t phs = (ph[kk] << 1) | (kk ? ph[kk > 0 ? kk - 1 : 0] >> 63 : (uint64_t)(carried & 1));
+            const uint64_t mhs = (mh[kk] << 1) | (kk ? mh[kk > 0 ? kk - 1 : 0] >> 63 : (uint64_t)((carried >> 1) & 1));
+            const uint64_t npv = mhs | ~(xv[kk] | phs);
+            const uint64_t nmv = phs & xv[kk];
+            if constexpr (TRACE) {
+                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
+                const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
+                const uint64_t plane0 = ~unequal_step & (npv | ~ph[kk]);
+                const uint64_t plane1 = ~unequal_step & ~npv;
+                if (active) {
+                    ulonglong2 *cell = reinterpret_cast<ulonglong2 *>(trace + ((size_t)kk * m + (j - 1)) * 16);
+                    *cell = make_ulonglong2(plane0, plane1);
+                }
+            }
+            pv[kk] = npv;
+            mv[kk] = nmv;
+        }
+        top += (int)(pv[0] & 1) - (int)(mv[0] & 1);
+    }
+}
+
+// ------------------------------------------------------------------ the walk
+
+// One thread a queued pair: from (n, m) back to the origin over the 2-bit decisions, then the CIGAR text.
+__global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int count = *a.walk_count;
+    if ((t & ~63) >= count) return;  // whole waves only: publishing takes every lane of a wave
+    const int slot = t < count ? a.walk_list[t] : -1;
+    int64_t moves = -1;
+    if (slot >= 0) {
+        const int n = a.s.tape.rows[slot], m = a.s.tape.cols[slot];
+        const int bin = a.s.walk_class[slot];
+        const int hi = a.s.walk_low[slot] + edit_width(bin) - 1;
+        const uint8_t *base = a.arena + (n > 0 && m > 0 ? a.trace_offset[slot] : 0);
+        OpTape ops = op_tape(a.s, slot);
+        const uint32_t unequal = a.s.eqx ? 1u : 0u;
+        const int width = edit_width(bin);
+        int r = n, c = m;
+        while (r > 0 && c > 0) {
+            const int i = hi - c + r;
+            if (i < 0 || i >= width) {  // a proven band holds every step; this pair's did not
+                a.s.status[slot] = 3;
+                break;
+            }
+            const uint64_t *cell = reinterpret_cast<const uint64_t *>(base + ((size_t)(i >> 6) * m + (c - 1)) * 16);
+            const uint32_t bit = i & 63;
+            const uint32_t code = (uint32_t)((cell[0] >> bit) & 1) | ((uint32_t)((cell[1] >> bit) & 1) << 1);
+            if (code == 1) {
+                ops.put(2u);
+                --r;
+            } else if (code == 2) {
+                ops.put(3u);
+                --c;
+            } else {
+                ops.put(code == 0 ? unequal : 0u);
+                --r;
+                --c;
+            }
+        }
+        for (; r > 0; --r) ops.put(2u);
+        for (; c > 0; --c) ops.put(3u);
+        ops.close();
+        moves = ops.count;
+    }
+    publish_cigars(a.s, slot >= 0 ? slot : 0, moves);
+}
+
+// ------------------------------------------------------------------ lists
+
+// The start of a run: every slot unsettled, the first queues in place, every count and the arena empty.
+__global__ void edit_reset(EditArgs a, int slots) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < slots) {
+        a.s.status[t] = -1;
+        a.band_low[t] = a.first_band_low[t];
+    }
+    for (int bin = 0; bin < EDIT_BINS; ++bin)
+        if (t < a.first_counts[bin])
+            a.lists[(size_t)bin * a.capacity + t] = a.first_lists[(size_t)bin * a.capacity + t];
+    if (t < EDIT_BINS) {
+        a.counts[t] = a.first_counts[t];
+        a.counts[EDIT_BINS + t] = 0;
+        a.trace_counts[t] = 0;
+        a.trace_counts[EDIT_BINS + t] = 0;
+    }
+    if (t <= 2 * EDIT_BINS) a.stats[t] = 0;
+    if (t == 0) {
+        *a.walk_count = 0;
+        *a.deferred_count = 0;
+        *a.arena_used = 0;
+        if (a.s.cigar_used) *a.s.cigar_used = 0;
+    }
+}
+
+// A further tracing pass: the arena and the trace and walk queues emptied (one workgroup, before the requeue).
+__global__ void edit_clear(EditArgs a) {
+    const int t = threadIdx.x;
+    if (t < 2 * EDIT_BINS) a.trace_counts[t] = 0;
+    if (t == 0) {
+        *a.walk_count = 0;
+        *a.deferred_count = 0;
+        *a.arena_used = 0;
+    }
+}
+
+// Queues the pairs `held` (deferred by the pass before, `count` of them) for tracing, in the band `settle` chose,
+// deferring again those the arena has no room for.
+__global__ void edit_requeue(EditArgs a, const int32_t *held, int count) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const int slot = held[t];
+    const int bin = a.s.walk_class[slot];
+    const unsigned long long bytes =
+        (unsigned long long)a.s.tape.cols[slot] * edit_lanes(bin) * edit_words(bin) * 16ull;
+    const unsigned long long at = atomicAdd(a.arena_used, bytes);
+    if (at + bytes > a.arena_capacity) {
+        a.deferred[atomicAdd(a.deferred_count, 1)] = slot;
+        return;
+    }
+    a.trace_offset[slot] = (int64_t)at;
+    a.trace_lists[(size_t)bin * a.capacity + atomicAdd(&a.trace_counts[bin], 1)] = slot;
+}
+
+template <int L, int K, bool TRACE>
+void sweep_bits(const EditArgs &a, int bin, int grid, hipStream_t stream) {
+    if (a.s.tape.bits == 2)
+        hipLaunchKernelGGL((edit_sweep<L, K, 2, TRACE>), dim3(grid), dim3(256), 0, stream, a, bin);
+    else
+        hipLaunchKernelGGL((edit_sweep<L, K, 8, TRACE>), dim3(grid), dim3(256), 0, stream, a, bin);
+}
+
+template <bool TRACE>
+void sweep_bin(const EditArgs &a, int bin, int grid, hipStream_t stream) {
+    switch (bin) {
+        case 0: sweep_bits<1, 1, TRACE>(a, bin, grid, stream); break;
+        case 1: sweep_bits<2, 1, TRACE>(a, bin, grid, stream); break;
+        case 2: sweep_bits<4, 1, TRACE>(a, bin, grid, stream); break;
+        case 3: sweep_bits<8, 1, TRACE>(a, bin, grid, stream); break;
+        case 4: sweep_bits<16, 1, TRACE>(a, bin, grid, stream); break;
+        case 5: sweep_bits<32, 1, TRACE>(a, bin, grid, stream); break;
+        case 6: sweep_bits<64, 1, TRACE>(a, bin, grid, stream); break;
+        case 7: sweep_bits<64, 2, TRACE>(a, bin, grid, stream); break;
+        default: sweep_bits<64, 4, TRACE>(a, bin, grid, stream); break;
+    }
+}
+
+}  // namespace
+
+void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream) {
+    const int threads = slots > 2 * EDIT_BINS + 1 ? slots : 2 * EDIT_BINS + 1;
+    hipLaunchKernelGGL(edit_reset, dim3((threads + 255) / 256), dim3(256), 0, stream, a, slots);
+}
+
+void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream) {
+    if (trace)
+        sweep_bin<true>(a, bin, grid, stream);
+    else
+        sweep_bin<false>(a, bin, grid, stream);
+}
+
+void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {
+    hipLaunchKernelGGL(edit_walk, dim3((slots + 255) / 256), dim3(256), 0, stream, a);
+}
+
+void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream) {
+    hipLaunchKernelGGL(edit_clear, dim3(1), dim3(64), 0, stream, a);
+    if (count > 0)
+        hipLaunchKernelGGL(edit_requeue, dim3((count + 255) / 256), dim3(256), 0, stream, a, held, count);
+}
+
+}  // namespace dinara

@@ -125,6 +125,14 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_seed_learned.restype = i64
     handle.dinara_batch_set_retry.argtypes = [vp, i64]
     handle.dinara_batch_set_retry.restype = i64
+    handle.dinara_batch_set_graph.argtypes = [vp, i64]
+    handle.dinara_batch_set_graph.restype = i64
+    handle.dinara_batch_set_persistent.argtypes = [vp, i64]
+    handle.dinara_batch_set_persistent.restype = i64
+    handle.dinara_batch_set_tuning.argtypes = [vp, i64, vp]
+    handle.dinara_batch_set_tuning.restype = i64
+    handle.dinara_batch_autotune.argtypes = [vp, vp]
+    handle.dinara_batch_autotune.restype = i64
     handle.dinara_mlp_new.argtypes = [i64, ctypes.c_uint64]
     handle.dinara_mlp_new.restype = vp
     handle.dinara_mlp_free.argtypes = [vp]

@@ -202,7 +202,11 @@ void dinara_batch_free(dinara_batch *batch);
  * 5 the walk-back kernel (ms); for band class c 100+c its kernels' time (ms), 200+c its launches, 400+c the pairs
  * it swept. Classes: 0-3 sixteen lanes a pair (32/64/128/256 diagonals), 4-9 one wave a pair (64..2048, every
  * pair under a user band), 10-21 one pair a block of waves (512, 1024, 1536, 2048, 3072, 4096..8192 in steps of
- * 1024, 12288, 16384; the last two distances only). For roofline reporting. No counterpart in the reference. */
+ * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
+ * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
+ * needed; for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
+ * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began. For roofline
+ * reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
 /*
@@ -231,6 +235,29 @@ int64_t dinara_batch_seed_learned(dinara_batch *batch, dinara_mlp *net, double m
 /* How an unproven pair's next band is chosen: 0 (the default) the band its found cost proves, 1 plain band doubling,
  * the same band twice as wide (the policy the learned band is measured against). */
 int64_t dinara_batch_set_retry(dinara_batch *batch, int64_t doubling);
+
+/*
+ * New: the unit-cost path. A batch under unit costs (mismatch = extension, no opening, deletions priced the same,
+ * or all of it times one scale) with no user band is served by a banded bit-parallel sweep: pairs binned by band
+ * width (64 x 2^b diagonals), each bin one persistent launch in which groups of lanes take pairs from a queue as
+ * they finish, a pair the band cannot prove filed into a wider bin; alignments sweep each settled pair's band again
+ * keeping 2 bits a cell, then walk. No counterpart in the reference (its device edit distance is
+ * device_edit.mojo:29-207, whole matrices, distances only).
+ */
+#define DINARA_EDIT_LENGTH_CLASSES 5
+
+/* Runs replay one captured hipGraph of the run's launches (1, the default) or make them one by one with an event
+ * pair around each, for per-kernel times (0). */
+int64_t dinara_batch_set_graph(dinara_batch *batch, int64_t on);
+/* Lane groups refill from their bin's queue (1, the default) or sweep one pair each (0: a launch as wide as the
+ * bin's pairs, for comparison). */
+int64_t dinara_batch_set_persistent(dinara_batch *batch, int64_t on);
+/* The launch shape: persistent workgroups a compute unit (1..64), and for each class of the pairs' longer side
+ * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..8); null keeps them. */
+int64_t dinara_batch_set_tuning(dinara_batch *batch, int64_t grid_per_cu, const int64_t *start_bins);
+/* Chooses that shape by timing the batch's own first pass (workgroups a compute unit, then each class's start bin)
+ * and keeps it; writes the choice to `chosen` (1 + DINARA_EDIT_LENGTH_CLASSES values) unless null. */
+int64_t dinara_batch_autotune(dinara_batch *batch, int64_t *chosen);
 
 dinara_mlp *dinara_mlp_new(int64_t gpu, uint64_t seed);
 void dinara_mlp_free(dinara_mlp *net);

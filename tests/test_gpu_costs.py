@@ -208,11 +208,14 @@ def test_block_bands_one_and_several_waves():
     # Length differences that put the first band past one wave's.
     refs += ["ACGT" * 700, "G" * 2500, "ACG" * 900]
     qrys += ["ACGT" * 40, "GT" * 200, "TTAC" * 300]
-    for costs in (da.Costs.affine(4, 6, 2), da.Costs.edit()):
+    for costs in (da.Costs.affine(4, 6, 2), da.Costs.linear(3, 2)):
         swept = class_pairs(refs, qrys, costs, 1)
         assert sum(swept[10:22]) > 0 and sum(swept[12:22]) > 0, swept  # blocks of several waves ran
         check_alignments(refs, qrys, costs)
         check_distances(refs, qrys, costs)
+    # Unit costs take the bit-parallel path (test_gpu_edit.py); the same pairs there.
+    check_alignments(refs, qrys, da.Costs.edit())
+    check_distances(refs, qrys, da.Costs.edit())
 
 
 def test_widest_band_distances():

@@ -1,0 +1,270 @@
+"""GPU parity of the unit-cost path (the banded bit-parallel sweep, edit.hip) through the C ABI: distances and
+CIGARs against the oracle bit for bit, on bands of one lane up to 64 lanes of four words; and every launch shape
+the batch can take (the captured graph or launches one by one, refilling lane groups or one pair a group, band
+doubling or the proven band, a tracing arena too small for the batch, the autotuner's choice) changes no result.
+
+The unit-cost CIGAR is the reference's bit-parallel traceback rule (traceback.mojo:400-575), which at unit costs
+equals its DP-flag rule (lanes.mojo:905-923); the oracle follows the flag rule and is pinned against both.
+"""
+
+import ctypes
+import os
+import random
+
+import numpy as np
+import pytest
+
+import dinara_align as da
+from dinara_align import _lib, synthetic
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+EDIT = o.Costs.affine(1, 0, 1)
+
+
+def mutated(text, rate, rng, alphabet="ACGT"):
+    out = []
+    for ch in text:
+        u = rng.random()
+        if u < rate / 2:
+            out.append(rng.choice([c for c in alphabet if c != ch]))
+        elif u < 3 * rate / 4:
+            out.append(rng.choice(alphabet))
+            out.append(ch)
+        elif u < rate:
+            continue
+        else:
+            out.append(ch)
+    return "".join(out)
+
+
+def pairs_of(count, rng, lengths, rates, alphabet="ACGT"):
+    refs, qrys = [], []
+    for i in range(count):
+        r = "".join(rng.choice(alphabet) for _ in range(rng.randint(*lengths)))
+        refs.append(r)
+        qrys.append(mutated(r, rates[i % len(rates)], rng, alphabet))
+    return refs, qrys
+
+
+class Batch:
+    """A resident batch under unit costs (or `costs`), with the unit path's switches."""
+
+    def __init__(self, refs, qrys, kind, costs=(1, 0, 1), ties_right=False, eqx=True, max_cost=-1):
+        self.h = _lib.lib()
+        self.R, self.Q = _lib.Texts(refs), _lib.Texts(qrys)
+        mismatch, opening, extension = costs
+        self.costs = _lib.CCosts(mismatch, opening, extension, -1, 0, 0, 0, -1, 0)
+        self.options = _lib.COptions(-(1 << 60), 1 << 60, max_cost, 1 if eqx else 0, 1 if ties_right else 0,
+                                     80 << 20)
+        status = ctypes.c_int64(0)
+        self.n = len(refs)
+        self.kind = kind
+        self.ptr = self.h.dinara_batch_new(0, kind, self.n, *self.R.args(), *self.Q.args(), ctypes.byref(self.costs),
+                                           None, ctypes.byref(self.options), 4, ctypes.byref(status))
+        assert self.ptr, status.value
+
+    def stat(self, what):
+        return self.h.dinara_batch_stat(self.ptr, what)
+
+    def run(self):
+        assert self.h.dinara_batch_run(self.ptr) == 0
+        if self.kind == 0:
+            out = (ctypes.c_int64 * self.n)()
+            assert self.h.dinara_batch_distances(self.ptr, out) == 0
+            return list(out)
+        found = (_lib.CAlignment * self.n)()
+        statuses = (ctypes.c_int64 * self.n)()
+        assert self.h.dinara_batch_alignments(self.ptr, found, statuses) == 0
+        out = []
+        for i in range(self.n):
+            if statuses[i] == 0:
+                out.append((found[i].cost, ctypes.string_at(found[i].cigar, found[i].cigar_length).decode()))
+                self.h.dinara_free(found[i].cigar)
+            else:
+                out.append(statuses[i])
+        return out
+
+    def free(self):
+        self.h.dinara_batch_free(self.ptr)
+
+
+def oracle_alignments(refs, qrys, right=False, eqx=True):
+    return [o.costs_align(r, q, EDIT, (o.NO_BAND_LOW, o.NO_BAND_HIGH), right_ties=right, eqx=eqx)
+            for r, q in zip(refs, qrys)]
+
+
+def test_unit_costs_take_the_bit_parallel_path():
+    refs, qrys = ["ACGTACGTTTGCA", "ACGT"], ["ACGTCGTTTTGCA", "AGT"]
+    b = Batch(refs, qrys, 1)
+    try:
+        assert b.stat(6) == 1
+        assert b.run() == [(2, "4=1D2=1I6="), (1, "1=1D2=")]
+        assert b.stat(7) == 1  # the run replayed its captured graph
+    finally:
+        b.free()
+    affine = Batch(refs, qrys, 0, costs=(4, 6, 2))
+    try:
+        assert affine.stat(6) == 0
+    finally:
+        affine.free()
+
+
+@pytest.mark.parametrize("right", [False, True])
+def test_bins_of_several_lanes_against_the_oracle(right):
+    # 64 to 4096 diagonals: pairs of 300..2500 letters at up to 50% edits take bins of 1..64 lanes.
+    rng = random.Random(31 + right)
+    refs, qrys = pairs_of(36, rng, (300, 2500), (0.01, 0.05, 0.15, 0.3, 0.5))
+    b = Batch(refs, qrys, 1, ties_right=right)
+    try:
+        got = b.run()
+        swept = [b.stat(700 + k) for k in range(9)]
+    finally:
+        b.free()
+    assert got == oracle_alignments(refs, qrys, right=right)
+    assert sum(1 for k in range(2, 7) if swept[k] > 0) >= 3, swept
+
+
+def test_words_of_two_and_four_a_lane():
+    # Unrelated sequences of 5,000..9,000 letters: distances near half their length need bands of 8192 and 16384
+    # diagonals (64 lanes of two and four words).
+    rng = random.Random(37)
+    refs = ["".join(rng.choice("ACGT") for _ in range(rng.randint(5000, 9000))) for _ in range(4)]
+    qrys = ["".join(rng.choice("ACGT") for _ in range(rng.randint(5000, 9000))) for _ in range(4)]
+    b = Batch(refs, qrys, 0)
+    try:
+        got = b.run()
+        swept = [b.stat(700 + k) for k in range(9)]
+    finally:
+        b.free()
+    assert got == [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
+    assert swept[7] + swept[8] > 0, swept
+    # CIGARs through bands of two words a lane: a length difference of 5000 needs more than 4096 diagonals.
+    r = refs[0][:6000]
+    qs = [mutated(r[1000:2000], 0.1, rng), "".join(rng.choice("ACGT") for _ in range(900))]
+    b = Batch([r, r], qs, 1)
+    try:
+        got = b.run()
+        traced = [b.stat(900 + k) for k in range(9)]
+    finally:
+        b.free()
+    assert got == oracle_alignments([r, r], qs)
+    assert traced[7] + traced[8] == 2, traced
+
+
+def test_byte_letters_and_scaled_costs():
+    rng = random.Random(41)
+    refs, qrys = pairs_of(60, rng, (0, 400), (0.05, 0.3), alphabet="ACDEFGHIKLMNPQRSTVWY")
+    b = Batch(refs, qrys, 1)
+    try:
+        assert b.run() == oracle_alignments(refs, qrys)
+    finally:
+        b.free()
+    # Costs three times unit: every cost three times, the same CIGARs.
+    refs, qrys = pairs_of(60, rng, (0, 300), (0.05, 0.3))
+    b = Batch(refs, qrys, 1, costs=(3, 0, 3))
+    try:
+        got = b.run()
+    finally:
+        b.free()
+    want = oracle_alignments(refs, qrys)
+    assert got == [(3 * c, cigar) for c, cigar in want]
+
+
+def test_caps_and_edges():
+    rng = random.Random(43)
+    refs, qrys = pairs_of(80, rng, (0, 300), (0.0, 0.1, 0.4))
+    refs += ["", "A", "", "ACGT" * 30, "A" * 70]
+    qrys += ["", "", "C", "", "T" * 3]
+    want = [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
+    for cap in (0, 7, 40):
+        b = Batch(refs, qrys, 0, max_cost=cap)
+        try:
+            got = b.run()
+        finally:
+            b.free()
+        assert got == [w if w <= cap else -4 for w in want]  # DINARA_ABOVE_MAX
+        b = Batch(refs, qrys, 1, max_cost=cap)
+        try:
+            got = b.run()
+        finally:
+            b.free()
+        oracle = oracle_alignments(refs, qrys)
+        assert got == [w if w[0] <= cap else -4 for w in oracle]
+
+
+def uneven(count, seed):
+    return synthetic.uneven(count, seed=seed, clip=(50, 3000))
+
+
+def results_with(refs, qrys, kind, setup):
+    b = Batch(refs, qrys, kind)
+    try:
+        setup(b)
+        first = b.run()
+        again = b.run()  # a second run replays what the first captured
+        assert first == again
+        return first, b
+    finally:
+        b.free()
+
+
+def test_launch_shapes_change_no_result():
+    refs, qrys = uneven(400, 47)
+    want = oracle_alignments(refs, qrys)
+    h = _lib.lib()
+    shapes = {
+        "graph": lambda b: None,
+        "eager": lambda b: h.dinara_batch_set_graph(b.ptr, 0),
+        "one pair a group": lambda b: h.dinara_batch_set_persistent(b.ptr, 0),
+        "doubling": lambda b: h.dinara_batch_set_retry(b.ptr, 1),
+        "wide start": lambda b: h.dinara_batch_set_tuning(b.ptr, 2, (ctypes.c_int64 * 5)(3, 3, 4, 5, 6)),
+    }
+    for name, setup in shapes.items():
+        got, _ = results_with(refs, qrys, 1, setup)
+        assert got == want, name
+
+
+def test_small_arena_traces_in_several_passes(monkeypatch):
+    refs, qrys = uneven(300, 53)
+    want = oracle_alignments(refs, qrys)
+    monkeypatch.setenv("DINARA_TRACE_ARENA", str(1 << 20))
+    b = Batch(refs, qrys, 1)
+    try:
+        got = b.run()
+        passes = b.stat(8)
+        again = b.run()
+    finally:
+        b.free()
+    assert passes >= 2
+    assert got == want and again == want
+
+
+def test_autotune_keeps_every_result():
+    refs, qrys = uneven(600, 59)
+    h = _lib.lib()
+    b = Batch(refs, qrys, 1)
+    try:
+        chosen = (ctypes.c_int64 * 6)()
+        assert h.dinara_batch_autotune(b.ptr, chosen) == 0
+        assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < 9 for i in range(1, 6))
+        got = b.run()
+    finally:
+        b.free()
+    assert got == oracle_alignments(refs, qrys)
+
+
+def test_seeds_change_no_result():
+    refs, qrys = uneven(300, 61)
+    want = [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
+    h = _lib.lib()
+    rng = np.random.default_rng(3)
+    for guess in (np.array(want) * 2, np.zeros(len(want)), rng.integers(0, 50, len(want))):
+        b = Batch(refs, qrys, 0)
+        try:
+            seeds = np.ascontiguousarray(guess, dtype=np.int64)
+            assert h.dinara_batch_seed(b.ptr, seeds.ctypes.data) == 0
+            assert b.run() == want
+        finally:
+            b.free()
