@@ -163,9 +163,10 @@ def band_cells(n, m, low, high):
     return pos + neg
 
 
-def proven_band_cells(n, m, cost, strict):
-    """The cells of the narrowest band the certificate proves at each pair's true cost under (4, 6, 2): out from
-    the start and end diagonals while a path off the band could cost less (no more, `strict`) than the pair."""
+def proven_band_cells(n, m, cost, strict, opening=OPEN, extension=EXTEND):
+    """The cells of the narrowest band the certificate proves at each pair's true cost under gap costs (opening,
+    extension), (4, 6, 2) unless given: out from the start and end diagonals while a path off the band could cost
+    less (no more, `strict`) than the pair."""
     n = n.astype(np.int64)
     m = m.astype(np.int64)
     cost = cost.astype(np.int64)
@@ -174,7 +175,7 @@ def proven_band_cells(n, m, cost, strict):
     def off(k):
         up = k > np.maximum(0, end)
         down = k < np.minimum(0, end)
-        gap = lambda x: np.where(x > 0, OPEN + x * EXTEND, 0)
+        gap = lambda x: np.where(x > 0, opening + x * extension, 0)
         return np.where(up, gap(k) + gap(k - end), np.where(down, gap(-k) + gap(end - k), 0))
 
     def grow(start, step, limit):
@@ -487,19 +488,345 @@ def run_local(args) -> int:
     return 0
 
 
+EDIT_BINS = 9
+EDIT_WIDTHS = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+BIT_OPS_FIRST = 34   # int32 ops a 64-cell word of the bit-parallel step (17 on 64-bit words)
+BIT_OPS_TRACE = 56   # the same with the 2-bit decisions of the tracing sweep
+
+
+def load_edit_baseline():
+    lib, flags = compile_native("baseline_edit")
+    i64, vp = ctypes.c_int64, ctypes.c_void_p
+    lib.baseline_edit.argtypes = [i64, vp, vp, vp, vp, i64, i64, vp, vp, i64]
+    lib.baseline_edit.restype = i64
+    return lib, flags
+
+
+def edit_batch(handle, refs, qrys, kind, threads):
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    costs = _lib.CCosts(1, 0, 1, -1, 0, 0, 0, -1, 0)
+    status = ctypes.c_int64(0)
+    b = handle.dinara_batch_new(0, kind, len(refs), *R.args(), *Q.args(), ctypes.byref(costs), None, None, threads,
+                                ctypes.byref(status))
+    if not b:
+        raise RuntimeError(f"dinara_batch_new failed: {status.value}")
+    if handle.dinara_batch_stat(b, 6) != 1:
+        raise RuntimeError("the unit-cost batch did not take the bit-parallel path")
+    return b, (R, Q)
+
+
+def edit_results(handle, b, kind, count):
+    """(costs, CIGARs) of the last run, as arrays / bytes."""
+    if kind == _lib.BATCH_DISTANCES:
+        out = (ctypes.c_int64 * count)()
+        if handle.dinara_batch_distances(b, out) != 0:
+            raise RuntimeError("distances failed")
+        return np.array(out[:], dtype=np.int64), None
+    found = (_lib.CAlignment * count)()
+    statuses = (ctypes.c_int64 * count)()
+    if handle.dinara_batch_alignments(b, found, statuses) != 0:
+        raise RuntimeError("alignments failed")
+    costs = np.array([found[i].cost if statuses[i] == 0 else -1 for i in range(count)], dtype=np.int64)
+    cigars = [ctypes.string_at(found[i].cigar, found[i].cigar_length) if statuses[i] == 0 else b""
+              for i in range(count)]
+    for i in range(count):
+        if statuses[i] == 0:
+            handle.dinara_free(found[i].cigar)
+    return costs, cigars
+
+
+def timed_steps(handle, b, steps, warmup, before=None):
+    """Wall ms of each of `steps` runs after `warmup` (each run preceded by `before()` when given, inside the
+    timed region), and the device ms the library's events saw."""
+    for _ in range(warmup):
+        if before:
+            before()
+        if handle.dinara_batch_run(b) != 0:
+            raise RuntimeError("warm-up run failed")
+    walls, device = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        if before:
+            before()
+        if handle.dinara_batch_run(b) != 0:
+            raise RuntimeError("run failed")
+        walls.append((time.perf_counter() - t0) * 1e3)
+        device.append(handle.dinara_batch_stat(b, 0))
+    return float(np.mean(walls)), float(np.mean(device))
+
+
+def bin_profile(handle, b):
+    """One run launched kernel by kernel (not the graph): each bin's first-pass and tracing time and sweeps."""
+    handle.dinara_batch_set_graph(b, 0)
+    if handle.dinara_batch_run(b) != 0:
+        raise RuntimeError("profiling run failed")
+    prof = {k: [handle.dinara_batch_stat(b, base + i) for i in range(EDIT_BINS)]
+            for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900))}
+    prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
+    prof["run_ms"] = handle.dinara_batch_stat(b, 0)
+    handle.dinara_batch_set_graph(b, 1)
+    return prof
+
+
+def run_uneven(args) -> int:
+    """Unit-cost distances and Ties.LEFT CIGARs of an uneven batch (the mixed-length workload): plain GPU band
+    doubling (each pair's band from 64 diagonals, twice as wide until proven; one pair a lane group, launches one
+    by one) against the persistent length-binned path (lane groups refilling from each bin's queue, the run one
+    captured hipGraph, start bins and workgroups autotuned on the batch) and against the same with learned band
+    seeds (the network trained on the aligner's own distances of separate pairs). Every variant's results are
+    compared byte for byte."""
+    pairs = args.pairs or 200_000
+    train_n = args.train if args.train is not None else 1_000_000
+    cores = host_cores()
+    workers = min(cores, 16)
+    t0 = time.time()
+    refs, qrys = synthetic.uneven_parallel(pairs, seed=3, workers=workers)
+    n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
+    m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
+    print(f"[bench] uneven workload ready: {pairs} pairs in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    handle = _lib.lib()
+    if handle.dinara_device_count() < 1:
+        print("[bench] no GPU visible", file=sys.stderr)
+        return 1
+    A = _lib.BATCH_ALIGNMENTS
+    results = {}
+
+    # Plain band doubling.
+    plain, keep_plain = edit_batch(handle, refs, qrys, A, workers)
+    handle.dinara_batch_set_retry(plain, 1)
+    handle.dinara_batch_set_persistent(plain, 0)
+    handle.dinara_batch_set_graph(plain, 0)
+    plain_ms, plain_dev = timed_steps(handle, plain, args.steps, args.warmup)
+    results["plain"] = edit_results(handle, plain, A, pairs)
+    plain_prof = bin_profile(handle, plain)
+    handle.dinara_batch_free(plain)
+    del keep_plain
+    print(f"[bench] plain doubling {plain_ms:.1f} ms/step", file=sys.stderr, flush=True)
+
+    # Persistent, captured, autotuned.
+    batch, keep = edit_batch(handle, refs, qrys, A, workers)
+    chosen = (ctypes.c_int64 * 6)()
+    t0 = time.time()
+    if handle.dinara_batch_autotune(batch, chosen) != 0:
+        raise RuntimeError("autotune failed")
+    tune_s = time.time() - t0
+    persistent_ms, persistent_dev = timed_steps(handle, batch, args.steps, args.warmup)
+    results["persistent"] = edit_results(handle, batch, A, pairs)
+    prof = bin_profile(handle, batch)
+    first_proven = handle.dinara_batch_stat(batch, 4)
+    print(f"[bench] persistent {persistent_ms:.1f} ms/step (tuned in {tune_s:.1f}s: {list(chosen)})",
+          file=sys.stderr, flush=True)
+
+    # Learned seeds: a network trained on the aligner's own distances of separate pairs.
+    learned_ms = learned_dev = None
+    learned_info = None
+    if train_n > 0:
+        t0 = time.time()
+        train_r, train_q = synthetic.uneven_parallel(train_n, seed=41, workers=workers)
+        tb, tkeep = edit_batch(handle, train_r, train_q, _lib.BATCH_DISTANCES, workers)
+        if handle.dinara_batch_run(tb) != 0:
+            raise RuntimeError("labelling run failed")
+        labels, _ = edit_results(handle, tb, _lib.BATCH_DISTANCES, train_n)
+        x_train = np.zeros((train_n, 288), dtype=np.float32)
+        if handle.dinara_batch_features(tb, x_train.ctypes.data) != 0:
+            raise RuntimeError("features failed")
+        handle.dinara_batch_free(tb)
+        del tkeep, train_r, train_q
+        y_train = np.log1p(np.maximum(labels, 0)).astype(np.float32)
+        net = handle.dinara_mlp_new(0, 17)
+        loss = ctypes.c_double(0)
+        epochs = args.epochs or 3
+        t1 = time.time()
+        if handle.dinara_mlp_train(net, train_n, x_train.ctypes.data, y_train.ctypes.data, epochs, 4096, 2e-3, 7,
+                                   ctypes.byref(loss)) != 0:
+            raise RuntimeError("training failed")
+        train_s = time.time() - t1
+        del x_train
+        margin = 0.0
+        seed = lambda: handle.dinara_batch_seed_learned(batch, net, ctypes.c_double(margin))
+        learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
+        results["learned"] = edit_results(handle, batch, A, pairs)
+        learned_proven = handle.dinara_batch_stat(batch, 4)
+        learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
+                        "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
+                        "margin": margin, "first_band_proved": round(learned_proven / pairs, 4)}
+        handle.dinara_mlp_free(net)
+        print(f"[bench] learned {learned_ms:.1f} ms/step", file=sys.stderr, flush=True)
+    handle.dinara_batch_free(batch)
+    del keep
+
+    costs, cigars = results["persistent"]
+    same_variants = all(np.array_equal(r[0], costs) and r[1] == cigars for r in results.values())
+    best_name = min((k for k in ("persistent", "learned") if k in results),
+                    key=lambda k: persistent_ms if k == "persistent" else learned_ms)
+    value_ms = learned_ms if learned_ms is not None else persistent_ms
+    value = pairs / (value_ms / 1e3)
+
+    # Roofline of the first pass (the persistent sweeps of every bin, the run's dominant phase): each pair's proven
+    # band at its true cost, BIT_OPS_FIRST int32 ops a 64-cell word, over the bins' event-timed launches.
+    proven = proven_band_cells(n, m, np.maximum(costs, 0), True, opening=0, extension=1)
+    first_ms = float(np.sum(prof["sweep_ms"]))
+    trace_ms = float(np.sum(prof["trace_ms"]))
+    first_ops = float(proven.sum()) * BIT_OPS_FIRST / 64
+    achieved = first_ops / (first_ms / 1e3) / 1e12
+    trace_achieved = float(proven.sum()) * BIT_OPS_TRACE / 64 / (trace_ms / 1e3) / 1e12 if trace_ms > 0 else None
+    roofline = {"bound": "valu", "kernel": "edit_sweep (first pass, every bin)", "achieved": round(achieved, 3),
+                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                "traffic": None, "launch_ms": round(first_ms, 3),
+                "trace_pass": {"achieved": round(trace_achieved, 3) if trace_achieved else None,
+                               "frac": round(trace_achieved / VALU_PEAK_TOPS, 4) if trace_achieved else None,
+                               "ms": round(trace_ms, 3)},
+                "note": "proven-band cells at the true cost (strict) x 34 int32 ops a 64-cell word (first pass), "
+                        "x 56 (tracing pass); HIP events around each bin's launch in a run launched kernel by kernel"}
+
+    cpu = None
+    exact_cpu = None
+    if not args.no_cpu:
+        lib, flags = load_edit_baseline()
+        rng = np.random.default_rng(11)
+        sample = np.sort(rng.choice(pairs, size=min(pairs, args.cpu_sample), replace=False))
+        s_refs = [refs[i] for i in sample]
+        s_qrys = [qrys[i] for i in sample]
+        R, Q = _lib.Texts(s_refs), _lib.Texts(s_qrys)
+        stride = int((n[sample] + m[sample]).max()) * 2 + 2
+        cpu_costs = (ctypes.c_int64 * len(sample))()
+        buf = ctypes.create_string_buffer(len(sample) * stride)
+        t0 = time.perf_counter()
+        lib.baseline_edit(len(sample), ctypes.cast(R.pointers, ctypes.c_void_p), ctypes.cast(R.lengths, ctypes.c_void_p),
+                          ctypes.cast(Q.pointers, ctypes.c_void_p), ctypes.cast(Q.lengths, ctypes.c_void_p), cores, 0,
+                          cpu_costs, buf, stride)
+        cpu_s = time.perf_counter() - t0
+        raw = buf.raw
+        same = sum(int(cpu_costs[k] == costs[i] and raw[k * stride:(k + 1) * stride].split(b"\0", 1)[0] == cigars[i])
+                   for k, i in enumerate(sample))
+        exact_cpu = same / len(sample)
+        cpu = {"value": round(len(sample) / cpu_s, 1), "unit": "pairs/s", "cores": cores, "kind": "port",
+               "sample": f"{len(sample)} pairs drawn uniformly from the workload, per-pair bit-parallel band "
+                         f"doubling without seeds (slower than the reference's A*PA2), {flags}, {cpu_model()}"}
+        print(f"[bench] CPU baseline {len(sample) / cpu_s:.0f} pairs/s on {cores} cores; equal on {same}/{len(sample)}",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "pairs/s + GCUPS on 1 MI355X: learned-band vs plain band doubling, bit-exact",
+        "value": round(value, 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(value_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "unit-cost distance + Ties.LEFT CIGAR, 200,000 pairs log-normal around 1.5 kbp (sigma 1, "
+                        "100 bp..20 kbp), 1-15% error (50/25/25), learned band seeds + persistent length-binned "
+                        "hipGraph launch, start bins and workgroups autotuned",
+            "pairs": pairs,
+        },
+        "gcups": round(float((n * m).sum()) / (value_ms / 1e3) / 1e9, 2),
+        "variants_ms_per_step": {"plain_doubling": round(plain_ms, 3), "persistent_tuned": round(persistent_ms, 3),
+                                 "learned_persistent": round(learned_ms, 3) if learned_ms else None},
+        "speedup_vs_plain_doubling": round(plain_ms / value_ms, 3),
+        "fastest_variant": best_name,
+        "device_ms_per_step": {"plain_doubling": round(plain_dev, 3), "persistent_tuned": round(persistent_dev, 3),
+                               "learned_persistent": round(learned_dev, 3) if learned_dev else None},
+        "bit_exact_across_variants": bool(same_variants),
+        "bit_exact_vs_cpu_sample": exact_cpu,
+        "first_band_proved": round(first_proven / pairs, 4),
+        "tuning": {"grid_per_cu": int(chosen[0]), "start_bins": [int(chosen[i]) for i in range(1, 6)],
+                   "tune_s": round(tune_s, 1)},
+        "learned": learned_info,
+        "bins": {"persistent": prof, "plain_doubling": plain_prof},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0 if same_variants else 1
+
+
+def run_edit(args) -> int:
+    """Unit-cost distances of 10,000 pairs of 150 bp at 2% (configuration 1's workload): the persistent path, one
+    captured graph a run, against the restated lanes CPU path (oracle/baseline.cpp at costs (1, 0, 1))."""
+    pairs = args.pairs or 10_000
+    refs, qrys = synthetic.edit_short(pairs)
+    n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
+    m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
+    handle = _lib.lib()
+    if handle.dinara_device_count() < 1:
+        print("[bench] no GPU visible", file=sys.stderr)
+        return 1
+    cores = host_cores()
+    D = _lib.BATCH_DISTANCES
+    batch, keep = edit_batch(handle, refs, qrys, D, min(cores, 16))
+    ms, dev = timed_steps(handle, batch, args.steps, args.warmup)
+    costs, _ = edit_results(handle, batch, D, pairs)
+    prof = bin_profile(handle, batch)
+    handle.dinara_batch_free(batch)
+    proven = proven_band_cells(n, m, costs, False, opening=0, extension=1)
+    first_ms = float(np.sum(prof["sweep_ms"]))
+    achieved = float(proven.sum()) * BIT_OPS_FIRST / 64 / (first_ms / 1e3) / 1e12
+    roofline = {"bound": "valu", "kernel": "edit_sweep (every bin)", "achieved": round(achieved, 4),
+                "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 5),
+                "traffic": None, "launch_ms": round(first_ms, 4),
+                "note": "proven-band cells x 34 int32 ops a 64-cell word; 10,000 short pairs fill few lanes, so the "
+                        "launch is latency-bound"}
+    cpu = exact = None
+    if not args.no_cpu:
+        lib, flags = load_baseline(cores)
+        R, Q = keep
+        cpu_costs = (ctypes.c_int64 * pairs)()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lib.baseline_alignments(pairs, ctypes.cast(R.pointers, ctypes.c_void_p),
+                                    ctypes.cast(R.lengths, ctypes.c_void_p), ctypes.cast(Q.pointers, ctypes.c_void_p),
+                                    ctypes.cast(Q.lengths, ctypes.c_void_p), 1, 0, 1, cores, cpu_costs, None, 0)
+        cpu_s = (time.perf_counter() - t0) / reps
+        exact = float(np.mean(np.array(cpu_costs[:]) == costs))
+        cpu = {"value": round(pairs / cpu_s, 1), "unit": "pairs/s", "cores": cores, "kind": "port",
+               "sample": f"the full workload x {reps}, the lanes path at costs (1, 0, 1), {flags}, {cpu_model()}"}
+    line = {
+        "metric": "pairs/s (global unit-cost distances, bit-exact)",
+        "value": round(pairs / (ms / 1e3), 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "global unit-cost distances, 10,000 pairs of 150 bp at 2% (equal sub/ins/del)",
+                   "pairs": pairs},
+        "gcups": round(float((n * m).sum()) / (ms / 1e3) / 1e9, 2),
+        "device_ms_per_step": round(dev, 4),
+        "bit_exact_vs_cpu": exact,
+        "bins": prof,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=None, help="pairs (default: the workload's own count)")
-    ap.add_argument("--workload", choices=("affine", "local", "learned"), default="affine",
+    ap.add_argument("--workload", choices=("affine", "local", "learned", "uneven", "edit"), default="affine",
                     help="affine: global gap-affine (4,6,2) CIGARs, 100k x 250 bp (the default); local: local "
                          "alignment under a 20-letter table, 50k x (3000 aa window, 300 aa query); learned: gap-affine "
                          "distances of 200k uneven pairs, the learned band against plain band doubling")
     ap.add_argument("--train", type=int, default=None, help="learned: training pairs (default 1,000,000)")
     ap.add_argument("--epochs", type=int, default=None, help="learned: training epochs (default 3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=20_000, help="uneven: pairs the CPU baseline times")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "
@@ -509,6 +836,10 @@ def main() -> int:
         return run_local(args)
     if args.workload == "learned":
         return run_learned(args)
+    if args.workload == "uneven":
+        return run_uneven(args)
+    if args.workload == "edit":
+        return run_edit(args)
     args.pairs = args.pairs or 100_000
 
     t0 = time.time()

@@ -200,6 +200,8 @@ struct dinara_batch {
     int start_bin[EDIT_LENGTH_CLASSES] = {};  // the least first bin, by the pair's longer side
     std::vector<int32_t> edit_first_low;  // per slot: its first band's lowest diagonal
     std::vector<int32_t> edit_first_bin;
+    int64_t sweep_reach[EDIT_BINS] = {};  // pairs a bin's sweep may meet: first bin no wider, widest band no narrower
+    int64_t trace_reach[EDIT_BINS] = {};  // pairs a bin's tracing sweep may meet
     DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
         trace_counts, trace_offset, walk_list, walk_count, deferred, deferred_count, deferred_held, edit_arena,
         arena_used, edit_stats;
@@ -212,6 +214,8 @@ struct dinara_batch {
 namespace {
 
 int64_t unit_lists(dinara_batch *b);
+int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const std::vector<int32_t> &counts,
+                     size_t cap);
 
 // Builds the tape: pairs by length, longest first, 64 to a group, each group's words transposed.
 int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qrys) {
@@ -517,6 +521,7 @@ int64_t unit_lists(dinara_batch *b) {
     std::vector<int32_t> lists(EDIT_BINS * cap, 0), counts(EDIT_BINS, 0);
     b->edit_first_low.assign(cap, 0);
     b->edit_first_bin.assign(cap, -1);
+    int64_t sweep_reach[EDIT_BINS] = {}, trace_reach[EDIT_BINS] = {};
     for (int64_t s = 0; s < slots; ++s) {
         const int n = b->rows[s], m = b->cols[s];
         const int lo = b->first_low[s], hi = b->first_high[s];
@@ -525,11 +530,40 @@ int64_t unit_lists(dinara_batch *b) {
         b->edit_first_low[s] = edit_place(lo, hi, edit_width(bin), n, m);
         b->edit_first_bin[s] = bin;
         lists[bin * cap + counts[bin]++] = (int32_t)s;
+        // A band placed inside the matrix's diagonals and n + m + 3 wide proves any pair, and no traced band is
+        // wider than n + m + 1: the bins a pair can reach.
+        const int widest = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
+        for (int k = bin; k <= std::max(bin, widest); ++k) ++sweep_reach[k];
+        for (int k = 0; k <= widest; ++k) ++trace_reach[k];
     }
-    if (b->edit_graph) {
+    bool same_reach = true;
+    for (int k = 0; k < EDIT_BINS; ++k) {
+        same_reach = same_reach && sweep_reach[k] == b->sweep_reach[k] && trace_reach[k] == b->trace_reach[k];
+        b->sweep_reach[k] = sweep_reach[k];
+        b->trace_reach[k] = trace_reach[k];
+    }
+    if (!same_reach && b->edit_graph) {  // the launches' shape changes
         (void)hipGraphExecDestroy(b->edit_graph);
         b->edit_graph = nullptr;
     }
+    // The captured graph reads these buffers where they were: it is kept unless one moves.
+    DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
+                            &b->edit_lists, &b->edit_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
+                            &b->walk_list, &b->walk_count, &b->deferred, &b->deferred_count, &b->deferred_held,
+                            &b->arena_used, &b->edit_stats};
+    void *before[sizeof(held) / sizeof(held[0])];
+    for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i) before[i] = held[i]->ptr;
+    const int64_t code = unit_buffers(b, lists, counts, cap);
+    for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i)
+        if (held[i]->ptr != before[i] && b->edit_graph) {
+            (void)hipGraphExecDestroy(b->edit_graph);
+            b->edit_graph = nullptr;
+        }
+    return code;
+}
+
+int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const std::vector<int32_t> &counts,
+                     size_t cap) {
     DeviceContext *ctx = b->ctx;
     if (!b->edit_first_lists.upload(ctx, lists.data(), lists.size() * 4) ||
         !b->edit_first_counts.upload(ctx, counts.data(), counts.size() * 4) ||
@@ -572,16 +606,18 @@ EditArgs edit_args(dinara_batch *b) {
     return e;
 }
 
-// Workgroups of bin `bin`'s sweep: enough for every pair at once, at most `grid_per_cu` a compute unit when the
-// groups refill from the queue.
-int edit_grid(const dinara_batch *b, int bin) {
-    const int64_t every = ((int64_t)b->slot_pair.size() * edit_lanes(bin) + 255) / 256;
+// Workgroups of bin `bin`'s sweep (or tracing sweep): enough for every pair it may meet at once, at most
+// `grid_per_cu` a compute unit when the groups refill from the queue; 0 when no pair can reach the bin.
+int edit_grid(const dinara_batch *b, int bin, bool trace) {
+    const int64_t reach = trace ? b->trace_reach[bin] : b->sweep_reach[bin];
+    if (reach == 0) return 0;
+    const int64_t every = (reach * edit_lanes(bin) + 255) / 256;
     if (!b->persistent) return (int)std::max<int64_t>(1, every);
     return (int)std::max<int64_t>(1, std::min<int64_t>(every, (int64_t)b->ctx->compute_units * b->grid_per_cu));
 }
 
 // The arena the traced bands' decisions take: the batch's bound (no traced band is wider than its whole matrix),
-// at most 32 GiB or half the free memory, at least 1 MiB. DINARA_TRACE_ARENA sets it in bytes.
+// at most 0.6 of the free memory, at least 1 MiB. DINARA_TRACE_ARENA sets it in bytes.
 size_t arena_bytes(const dinara_batch *b) {
     if (const char *env = std::getenv("DINARA_TRACE_ARENA")) return std::max<size_t>(std::strtoull(env, nullptr, 10), 4096);
     size_t bound = 0;
@@ -592,7 +628,7 @@ size_t arena_bytes(const dinara_batch *b) {
     }
     size_t free_bytes = 0, total = 0;
     (void)hipMemGetInfo(&free_bytes, &total);
-    const size_t most = std::min<size_t>(size_t(32) << 30, free_bytes / 2);
+    const size_t most = free_bytes / 10 * 6;
     return std::max<size_t>(std::min(bound, most), size_t(1) << 20);
 }
 
@@ -604,13 +640,13 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace)
     launch_edit_reset(a, slots, ctx->stream);
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
         if (timed) (void)hipEventRecord(ctx->event(2 * bin), ctx->stream);
-        launch_edit_sweep(a, bin, false, edit_grid(b, bin), ctx->stream);
+        if (const int grid = edit_grid(b, bin, false)) launch_edit_sweep(a, bin, false, grid, ctx->stream);
         if (timed) (void)hipEventRecord(ctx->event(2 * bin + 1), ctx->stream);
     }
     if (trace) {
         for (int bin = 0; bin < EDIT_BINS; ++bin) {
             if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin)), ctx->stream);
-            launch_edit_sweep(a, bin, true, edit_grid(b, bin), ctx->stream);
+            if (const int grid = edit_grid(b, bin, true)) launch_edit_sweep(a, bin, true, grid, ctx->stream);
             if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin) + 1), ctx->stream);
         }
         if (timed) (void)hipEventRecord(ctx->event(4 * EDIT_BINS), ctx->stream);
@@ -678,7 +714,8 @@ int64_t run_unit(dinara_batch *b, bool trace) {
                                ctx->stream) != hipSuccess)
                 return DINARA_OUT_OF_MEMORY;
             launch_edit_requeue(a, static_cast<const int32_t *>(b->deferred_held.ptr), deferred, ctx->stream);
-            for (int bin = 0; bin < EDIT_BINS; ++bin) launch_edit_sweep(a, bin, true, edit_grid(b, bin), ctx->stream);
+            for (int bin = 0; bin < EDIT_BINS; ++bin)
+                if (const int grid = edit_grid(b, bin, true)) launch_edit_sweep(a, bin, true, grid, ctx->stream);
             launch_edit_walk(a, (int)slots, ctx->stream);
             int32_t left = 0;
             if (hipPeekAtLastError() != hipSuccess || !b->deferred_count.read(ctx, &left, 4)) return DINARA_OUT_OF_MEMORY;

@@ -63,29 +63,31 @@ __device__ __forceinline__ void load_planes(const uint32_t *text, int n, int fir
     }
 }
 
-// One sequence read a letter at a time: the code of letter `at` (0 outside 0..n-1), then the next.
+// One sequence read a letter at a time from letter `from` >= 0 on: the code of letter `at` (0 past n - 1), then
+// the next. The word after the current one is loaded a word ahead, so a column never waits on its own load (and,
+// since stores count against the same counter, never on the stores before it either).
 template <int BITS>
 struct Cursor {
+    static constexpr int PER = 32 / BITS;
     const uint32_t *text;
     int n, at;
-    uint32_t word;
+    uint32_t cur, ahead;
 
+    __device__ uint32_t word(int w) const { return w * PER < n ? text[(size_t)w * 64] : 0u; }
     __device__ void begin(const uint32_t *t, int length, int from) {
-        constexpr int PER = 32 / BITS;
         text = t;
         n = length;
         at = from;
-        word = (at >= 0 && at < n) ? text[(size_t)(at / PER) * 64] : 0u;
+        cur = word(at / PER);
+        ahead = word(at / PER + 1);
     }
     __device__ uint32_t next() {
-        constexpr int PER = 32 / BITS;
-        uint32_t code = 0;
-        if (at >= 0 && at < n) {
-            const int within = at % PER;
-            if (within == 0) word = text[(size_t)(at / PER) * 64];
-            code = (word >> (BITS * within)) & ((BITS == 8) ? 0xFFu : 3u);
-        }
+        const uint32_t code = (cur >> (BITS * (at % PER))) & ((BITS == 8) ? 0xFFu : 3u);
         ++at;
+        if (at % PER == 0) {
+            cur = ahead;
+            ahead = word(at / PER + 1);
+        }
         return code;
     }
 };
@@ -257,6 +259,15 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             continue;
         }
 
+        // Columns until the wave's next pair end: no group claims or finishes before then.
+        int steps = slot >= 0 ? m - j : 0x7FFFFFFF;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int other = __shfl_xor(steps, d);
+            steps = other < steps ? other : steps;
+        }
+        const bool active = slot >= 0;
+        for (int step = 0; step < steps; ++step) {
         // One column. The window moves down a row: each word takes the bit of the word below; the band's bottom
         // takes +1 under it (Pv), and the new bottom row's letter.
         ++j;
@@ -325,7 +336,6 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
         uint32_t carried = (uint32_t)(ph[K - 1] >> 63) | ((uint32_t)(mh[K - 1] >> 63) << 1);
         carried = L > 1 ? wave_neighbour<true>(carried) : 0u;
         if (first_lane) carried = 1u;
-        const bool active = slot >= 0;
 #pragma unroll
         for (int kk = K - 1; kk >= 0; --kk) {
             const uint64_t phs = (ph[kk] << 1) | (kk ? ph[kk > 0 ? kk - 1 : 0] >> 63 : (uint64_t)(carried & 1));
@@ -346,6 +356,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             mv[kk] = nmv;
         }
         top += (int)(pv[0] & 1) - (int)(mv[0] & 1);
+        }
     }
 }
 

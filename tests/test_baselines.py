@@ -82,3 +82,29 @@ def test_scoring_baseline_matches_oracle():
         want_score, want_cigar, want_spans = o.scoring_local_align(r, q, scoring)
         got = cigars.raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
         assert (scores[i], got, tuple(spans[4 * i:4 * i + 4])) == (want_score, want_cigar, want_spans), i
+
+
+@pytest.mark.parametrize("right", [False, True])
+def test_edit_baseline_matches_oracle(right):
+    lib = load("libbaseline_edit.so")
+    i64, vp = ctypes.c_int64, ctypes.c_void_p
+    lib.baseline_edit.argtypes = [i64, vp, vp, vp, vp, i64, i64, vp, vp, i64]
+    lib.baseline_edit.restype = i64
+    refs, qrys = synthetic.uneven(120, seed=27 + right, clip=(0, 900))
+    rng = random.Random(5)
+    refs = list(refs) + [b"", b"A", b"", b"ACGT" * 40, bytes(rng.choice(b"ACGT") for _ in range(700))]
+    qrys = list(qrys) + [b"", b"", b"C", b"", bytes(rng.choice(b"ACGT") for _ in range(90))]
+    _, rp, rl = texts(refs)
+    _, qp, ql = texts(qrys)
+    costs = (ctypes.c_int64 * len(refs))()
+    stride = 4000
+    cigars = ctypes.create_string_buffer(stride * len(refs))
+    assert lib.baseline_edit(len(refs), rp, rl, qp, ql, 3, int(right), costs, cigars, stride) == len(refs)
+    edit = o.Costs.affine(1, 0, 1)
+    for i, (r, q) in enumerate(zip(refs, qrys)):
+        want = o.costs_align(r, q, edit, right_ties=right)
+        got = cigars.raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
+        assert (costs[i], got) == want, i
+    distances = (ctypes.c_int64 * len(refs))()
+    lib.baseline_edit(len(refs), rp, rl, qp, ql, 2, 0, distances, None, 0)
+    assert list(distances) == [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
