@@ -591,9 +591,10 @@ def run_uneven(args) -> int:
     A = _lib.BATCH_ALIGNMENTS
     results = {}
 
-    # Plain band doubling.
+    # Plain band doubling: every band from 64 diagonals keeps its decisions, twice as wide until proven.
     plain, keep_plain = edit_batch(handle, refs, qrys, A, workers)
     handle.dinara_batch_set_retry(plain, 1)
+    handle.dinara_batch_set_keep_first(plain, 1)
     handle.dinara_batch_set_persistent(plain, 0)
     handle.dinara_batch_set_graph(plain, 0)
     plain_ms, plain_dev = timed_steps(handle, plain, args.steps, args.warmup)
@@ -605,7 +606,7 @@ def run_uneven(args) -> int:
 
     # Persistent, captured, autotuned.
     batch, keep = edit_batch(handle, refs, qrys, A, workers)
-    chosen = (ctypes.c_int64 * 6)()
+    chosen = (ctypes.c_int64 * 7)()
     t0 = time.time()
     if handle.dinara_batch_autotune(batch, chosen) != 0:
         raise RuntimeError("autotune failed")
@@ -642,14 +643,31 @@ def run_uneven(args) -> int:
             raise RuntimeError("training failed")
         train_s = time.time() - t1
         del x_train
-        margin = 0.0
+        margin = args.margin
         seed = lambda: handle.dinara_batch_seed_learned(batch, net, ctypes.c_double(margin))
+        # The seeds choose each pair's first band: no start bin holds it wider.
+        handle.dinara_batch_set_tuning(batch, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
+        seed()
+        two_pass_ms, _ = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
+        results["learned_two_pass"] = edit_results(handle, batch, A, pairs)
+        # With seeds that mostly prove their pairs, sweeps keep their decisions: no second sweep. The launch shape
+        # (workgroups, bins side by side or not) is tuned again for it.
+        handle.dinara_batch_set_keep_first(batch, 1)
+        seed()
+        learned_chosen = (ctypes.c_int64 * 7)()
+        if handle.dinara_batch_autotune(batch, learned_chosen) != 0:
+            raise RuntimeError("autotune failed")
         learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
         results["learned"] = edit_results(handle, batch, A, pairs)
         learned_proven = handle.dinara_batch_stat(batch, 4)
+        walked_first = handle.dinara_batch_stat(batch, 9)
+        learned_prof = bin_profile(handle, batch)
         learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
                         "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
-                        "margin": margin, "first_band_proved": round(learned_proven / pairs, 4)}
+                        "margin": margin, "first_band_proved": round(learned_proven / pairs, 4),
+                        "walked_from_first_sweep": round(walked_first / pairs, 4),
+                        "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
+                        "tuning": {"grid_per_cu": int(learned_chosen[0]), "side_by_side": int(learned_chosen[6])}}
         handle.dinara_mlp_free(net)
         print(f"[bench] learned {learned_ms:.1f} ms/step", file=sys.stderr, flush=True)
     handle.dinara_batch_free(batch)
@@ -736,7 +754,7 @@ def run_uneven(args) -> int:
         "bit_exact_vs_cpu_sample": exact_cpu,
         "first_band_proved": round(first_proven / pairs, 4),
         "tuning": {"grid_per_cu": int(chosen[0]), "start_bins": [int(chosen[i]) for i in range(1, 6)],
-                   "tune_s": round(tune_s, 1)},
+                   "side_by_side": int(chosen[6]), "tune_s": round(tune_s, 1)},
         "learned": learned_info,
         "bins": {"persistent": prof, "plain_doubling": plain_prof},
         "roofline": roofline,
@@ -744,6 +762,7 @@ def run_uneven(args) -> int:
     }
     print(json.dumps(line))
     return 0 if same_variants else 1
+
 
 
 def run_edit(args) -> int:
@@ -827,6 +846,7 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=None, help="learned: training epochs (default 3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="uneven: pairs the CPU baseline times")
+    ap.add_argument("--margin", type=float, default=0.0, help="uneven: learned seeds e^(prediction + margin) - 1")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "

@@ -196,19 +196,24 @@ struct dinara_batch {
     int unit_scale = 1;
     bool use_graph = true;                // replay the run's launches as one captured hipGraph
     bool persistent = true;               // pair groups refill from their queue; else one pair a group
+    bool keep_first = false;              // alignments: every sweep keeps its decisions (one pass when proven)
+    bool side_by_side_bins = true;        // the bins' first queues (and tracing sweeps) on streams of their own
     int grid_per_cu = 8;                  // persistent workgroups a compute unit
     int start_bin[EDIT_LENGTH_CLASSES] = {};  // the least first bin, by the pair's longer side
     std::vector<int32_t> edit_first_low;  // per slot: its first band's lowest diagonal
     std::vector<int32_t> edit_first_bin;
-    int64_t sweep_reach[EDIT_BINS] = {};  // pairs a bin's sweep may meet: first bin no wider, widest band no narrower
+    int64_t first_count[EDIT_BINS] = {};  // pairs each bin's first queue holds
+    int64_t retry_reach[EDIT_BINS] = {};  // pairs a bin's retry queue may get: first bin narrower, widest no narrower
     int64_t trace_reach[EDIT_BINS] = {};  // pairs a bin's tracing sweep may meet
-    DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
+    DeviceBuffer features_d, predictions_d;  // learned seeding on the device
+    int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
+    DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
         trace_counts, trace_offset, walk_list, walk_count, deferred, deferred_count, deferred_held, edit_arena,
         arena_used, edit_stats;
     size_t arena_capacity = 0;
     hipGraphExec_t edit_graph = nullptr;
     double edit_ms[EDIT_BINS] = {}, trace_ms[EDIT_BINS] = {}, edit_sweeps[EDIT_BINS] = {},
-           trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0;
+           trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0, walked_first = 0;
 };
 
 namespace {
@@ -521,7 +526,7 @@ int64_t unit_lists(dinara_batch *b) {
     std::vector<int32_t> lists(EDIT_BINS * cap, 0), counts(EDIT_BINS, 0);
     b->edit_first_low.assign(cap, 0);
     b->edit_first_bin.assign(cap, -1);
-    int64_t sweep_reach[EDIT_BINS] = {}, trace_reach[EDIT_BINS] = {};
+    int64_t first_count[EDIT_BINS] = {}, retry_reach[EDIT_BINS] = {}, trace_reach[EDIT_BINS] = {};
     for (int64_t s = 0; s < slots; ++s) {
         const int n = b->rows[s], m = b->cols[s];
         const int lo = b->first_low[s], hi = b->first_high[s];
@@ -533,22 +538,26 @@ int64_t unit_lists(dinara_batch *b) {
         // A band placed inside the matrix's diagonals and n + m + 3 wide proves any pair, and no traced band is
         // wider than n + m + 1: the bins a pair can reach.
         const int widest = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
-        for (int k = bin; k <= std::max(bin, widest); ++k) ++sweep_reach[k];
+        ++first_count[bin];
+        for (int k = bin + 1; k <= widest; ++k) ++retry_reach[k];
         for (int k = 0; k <= widest; ++k) ++trace_reach[k];
     }
     bool same_reach = true;
     for (int k = 0; k < EDIT_BINS; ++k) {
-        same_reach = same_reach && sweep_reach[k] == b->sweep_reach[k] && trace_reach[k] == b->trace_reach[k];
-        b->sweep_reach[k] = sweep_reach[k];
+        same_reach = same_reach && first_count[k] == b->first_count[k] && retry_reach[k] == b->retry_reach[k] &&
+                     trace_reach[k] == b->trace_reach[k];
+        b->first_count[k] = first_count[k];
+        b->retry_reach[k] = retry_reach[k];
         b->trace_reach[k] = trace_reach[k];
     }
+    b->seeded_start[0] = -1;  // host-filed queues: the device seeding's shape no longer holds
     if (!same_reach && b->edit_graph) {  // the launches' shape changes
         (void)hipGraphExecDestroy(b->edit_graph);
         b->edit_graph = nullptr;
     }
     // The captured graph reads these buffers where they were: it is kept unless one moves.
     DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
-                            &b->edit_lists, &b->edit_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
+                            &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
                             &b->walk_list, &b->walk_count, &b->deferred, &b->deferred_count, &b->deferred_held,
                             &b->arena_used, &b->edit_stats};
     void *before[sizeof(held) / sizeof(held[0])];
@@ -569,10 +578,11 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->edit_first_counts.upload(ctx, counts.data(), counts.size() * 4) ||
         !b->edit_first_low_d.upload(ctx, b->edit_first_low.data(), cap * 4) || !b->edit_band_low.reserve(cap * 4) ||
         !b->edit_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_counts.reserve(2 * EDIT_BINS * 4) ||
+        !b->edit_retry_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_retry_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_offset.reserve(cap * 8) || !b->walk_list.reserve(cap * 4) || !b->walk_count.reserve(4) ||
         !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
-        !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 1) * 4))
+        !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4))
         return DINARA_OUT_OF_MEMORY;
     return 0;
 }
@@ -589,8 +599,12 @@ EditArgs edit_args(dinara_batch *b) {
     e.first_band_low = static_cast<const int32_t *>(b->edit_first_low_d.ptr);
     e.lists = static_cast<int32_t *>(b->edit_lists.ptr);
     e.counts = static_cast<int32_t *>(b->edit_counts.ptr);
+    e.file_lists = static_cast<int32_t *>(b->edit_retry_lists.ptr);
+    e.file_counts = static_cast<int32_t *>(b->edit_retry_counts.ptr);
     e.first_lists = static_cast<int32_t *>(b->edit_first_lists.ptr);
     e.first_counts = static_cast<const int32_t *>(b->edit_first_counts.ptr);
+    e.first_counts_out = static_cast<int32_t *>(b->edit_first_counts.ptr);
+    e.first_low_out = static_cast<int32_t *>(b->edit_first_low_d.ptr);
     e.trace_lists = static_cast<int32_t *>(b->trace_lists.ptr);
     e.trace_counts = static_cast<int32_t *>(b->trace_counts.ptr);
     e.trace_offset = static_cast<int64_t *>(b->trace_offset.ptr);
@@ -603,13 +617,13 @@ EditArgs edit_args(dinara_batch *b) {
     e.arena_capacity = b->arena_capacity;
     e.stats = static_cast<uint32_t *>(b->edit_stats.ptr);
     e.one_pair = b->persistent ? 0 : 1;
+    e.keep_first = b->keep_first && b->kind == DINARA_BATCH_ALIGNMENTS ? 1 : 0;
     return e;
 }
 
-// Workgroups of bin `bin`'s sweep (or tracing sweep): enough for every pair it may meet at once, at most
-// `grid_per_cu` a compute unit when the groups refill from the queue; 0 when no pair can reach the bin.
-int edit_grid(const dinara_batch *b, int bin, bool trace) {
-    const int64_t reach = trace ? b->trace_reach[bin] : b->sweep_reach[bin];
+// Workgroups of a bin's launch that may meet `reach` pairs: enough for every one at once, at most `grid_per_cu` a
+// compute unit when the groups refill from the queue; 0 when it meets none.
+int edit_grid(const dinara_batch *b, int bin, int64_t reach) {
     if (reach == 0) return 0;
     const int64_t every = (reach * edit_lanes(bin) + 255) / 256;
     if (!b->persistent) return (int)std::max<int64_t>(1, every);
@@ -624,34 +638,73 @@ size_t arena_bytes(const dinara_batch *b) {
     for (size_t s = 0; s < b->slot_pair.size(); ++s) {
         const int n = b->rows[s], m = b->cols[s];
         const int bin = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
-        bound += (size_t)m * edit_lanes(bin) * edit_words(bin) * 16;
+        bound += (size_t)edit_trace_bytes(m, bin);
     }
     size_t free_bytes = 0, total = 0;
     (void)hipMemGetInfo(&free_bytes, &total);
     const size_t most = free_bytes / 10 * 6;
+    if (b->keep_first) bound *= 3;  // every attempt keeps its decisions
     return std::max<size_t>(std::min(bound, most), size_t(1) << 20);
 }
 
-// The run's launches in order: reset, the sweep of every bin, then (alignments) the tracing sweep of every bin and
-// the walk. `timed` records an event pair around each launch (not under capture).
+// Timing events of a run launched kernel by kernel: a pair for each bin's first sweep, retry sweep and tracing
+// sweep, and for the walk.
+constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, EV_WALK = 6 * EDIT_BINS;
+
+// Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
+// stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
+// EV + 2 bin + 1 when `timed`.
+template <typename Use, typename Launch>
+void side_by_side(DeviceContext *ctx, int mark_base, bool timed, int ev, bool apart, Use use, Launch launch) {
+    (void)hipEventRecord(ctx->mark(mark_base), ctx->stream);
+    for (int bin = 0; bin < EDIT_BINS; ++bin) {
+        if (!use(bin)) continue;
+        if (!apart) {  // one after another on the main stream
+            if (timed) (void)hipEventRecord(ctx->event(ev + 2 * bin), ctx->stream);
+            launch(bin, ctx->stream);
+            if (timed) (void)hipEventRecord(ctx->event(ev + 2 * bin + 1), ctx->stream);
+            continue;
+        }
+        hipStream_t side = ctx->side(bin);
+        (void)hipStreamWaitEvent(side, ctx->mark(mark_base), 0);
+        if (timed) (void)hipEventRecord(ctx->event(ev + 2 * bin), side);
+        launch(bin, side);
+        if (timed) (void)hipEventRecord(ctx->event(ev + 2 * bin + 1), side);
+        (void)hipEventRecord(ctx->mark(mark_base + 1 + bin), side);
+        (void)hipStreamWaitEvent(ctx->stream, ctx->mark(mark_base + 1 + bin), 0);
+    }
+}
+
+// The run's launches: reset; every bin's first queue, the bins side by side; the retry queues, narrowest bin
+// first; then (alignments) every bin's tracing sweep side by side, and the walk. `timed` records an event pair
+// around each launch (not under capture).
 int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace) {
     DeviceContext *ctx = b->ctx;
     const int slots = (int)b->slot_pair.size();
     launch_edit_reset(a, slots, ctx->stream);
+    side_by_side(ctx, 0, timed, EV_FIRST, b->side_by_side_bins, [&](int bin) { return b->first_count[bin] > 0; },
+                 [&](int bin, hipStream_t st) {
+                     launch_edit_sweep(a, bin, false, edit_grid(b, bin, b->first_count[bin]), st);
+                 });
+    EditArgs retry = a;
+    retry.lists = a.file_lists;
+    retry.counts = a.file_counts;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
-        if (timed) (void)hipEventRecord(ctx->event(2 * bin), ctx->stream);
-        if (const int grid = edit_grid(b, bin, false)) launch_edit_sweep(a, bin, false, grid, ctx->stream);
-        if (timed) (void)hipEventRecord(ctx->event(2 * bin + 1), ctx->stream);
+        const int grid = edit_grid(b, bin, b->retry_reach[bin]);
+        if (!grid) continue;
+        if (timed) (void)hipEventRecord(ctx->event(EV_RETRY + 2 * bin), ctx->stream);
+        launch_edit_sweep(retry, bin, false, grid, ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(EV_RETRY + 2 * bin + 1), ctx->stream);
     }
     if (trace) {
-        for (int bin = 0; bin < EDIT_BINS; ++bin) {
-            if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin)), ctx->stream);
-            if (const int grid = edit_grid(b, bin, true)) launch_edit_sweep(a, bin, true, grid, ctx->stream);
-            if (timed) (void)hipEventRecord(ctx->event(2 * (EDIT_BINS + bin) + 1), ctx->stream);
-        }
-        if (timed) (void)hipEventRecord(ctx->event(4 * EDIT_BINS), ctx->stream);
+        side_by_side(ctx, EDIT_BINS + 1, timed, EV_TRACE, b->side_by_side_bins,
+                     [&](int bin) { return b->trace_reach[bin] > 0; },
+                     [&](int bin, hipStream_t st) {
+                         launch_edit_sweep(a, bin, true, edit_grid(b, bin, b->trace_reach[bin]), st);
+                     });
+        if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
         launch_edit_walk(a, slots, ctx->stream);
-        if (timed) (void)hipEventRecord(ctx->event(4 * EDIT_BINS + 1), ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(EV_WALK + 1), ctx->stream);
     }
     return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
 }
@@ -715,7 +768,8 @@ int64_t run_unit(dinara_batch *b, bool trace) {
                 return DINARA_OUT_OF_MEMORY;
             launch_edit_requeue(a, static_cast<const int32_t *>(b->deferred_held.ptr), deferred, ctx->stream);
             for (int bin = 0; bin < EDIT_BINS; ++bin)
-                if (const int grid = edit_grid(b, bin, true)) launch_edit_sweep(a, bin, true, grid, ctx->stream);
+                if (const int grid = edit_grid(b, bin, b->trace_reach[bin]))
+                    launch_edit_sweep(a, bin, true, grid, ctx->stream);
             launch_edit_walk(a, (int)slots, ctx->stream);
             int32_t left = 0;
             if (hipPeekAtLastError() != hipSuccess || !b->deferred_count.read(ctx, &left, 4)) return DINARA_OUT_OF_MEMORY;
@@ -730,27 +784,31 @@ int64_t run_unit(dinara_batch *b, bool trace) {
     (void)hipEventElapsedTime(&sweeps, ctx->start, ctx->sweep_stop);
     b->run_ms = total;
     b->sweep_ms = sweeps;
-    uint32_t stats[2 * EDIT_BINS + 1] = {};
+    uint32_t stats[2 * EDIT_BINS + 2] = {};
     if (!b->edit_stats.read(ctx, stats, sizeof(stats))) return DINARA_OUT_OF_MEMORY;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
         b->edit_sweeps[bin] = stats[bin];
         b->trace_sweeps[bin] = stats[EDIT_BINS + bin];
         if (!graph) {
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, ctx->event(2 * bin), ctx->event(2 * bin + 1));
-            b->edit_ms[bin] = ms;
-            if (trace) {
-                (void)hipEventElapsedTime(&ms, ctx->event(2 * (EDIT_BINS + bin)), ctx->event(2 * (EDIT_BINS + bin) + 1));
+            if (b->first_count[bin] > 0 &&
+                hipEventElapsedTime(&ms, ctx->event(EV_FIRST + 2 * bin), ctx->event(EV_FIRST + 2 * bin + 1)) == hipSuccess)
+                b->edit_ms[bin] += ms;
+            if (b->retry_reach[bin] > 0 &&
+                hipEventElapsedTime(&ms, ctx->event(EV_RETRY + 2 * bin), ctx->event(EV_RETRY + 2 * bin + 1)) == hipSuccess)
+                b->edit_ms[bin] += ms;
+            if (trace && b->trace_reach[bin] > 0 &&
+                hipEventElapsedTime(&ms, ctx->event(EV_TRACE + 2 * bin), ctx->event(EV_TRACE + 2 * bin + 1)) == hipSuccess)
                 b->trace_ms[bin] = ms;
-            }
         }
     }
     if (!graph && trace) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ctx->event(4 * EDIT_BINS), ctx->event(4 * EDIT_BINS + 1));
+        (void)hipEventElapsedTime(&ms, ctx->event(EV_WALK), ctx->event(EV_WALK + 1));
         b->walk_ms = ms;
     }
     b->first_proven = (double)(slots - stats[2 * EDIT_BINS]);
+    b->walked_first = stats[2 * EDIT_BINS + 1];
     b->launches = graph ? 1 : 1 + EDIT_BINS * (trace ? 2 : 1) + (trace ? 1 : 0);
     b->ran = true;
     return 0;
@@ -1134,6 +1192,57 @@ int64_t seed_slots(dinara_batch *b, GuessOf guess_of) {
     return 0;
 }
 
+// The launch shape when the device files each pair's first band: a first queue may hold any pair whose class starts
+// at or below the bin and whose widest band reaches it. Caller holds the device lock.
+void seeded_shape(dinara_batch *b) {
+    bool same = true;
+    for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) same = same && b->seeded_start[c] == b->start_bin[c];
+    if (same) return;
+    int64_t first[EDIT_BINS] = {}, retry[EDIT_BINS] = {}, trace[EDIT_BINS] = {};
+    for (size_t s = 0; s < b->slot_pair.size(); ++s) {
+        const int n = b->rows[s], m = b->cols[s];
+        const int start = b->start_bin[edit_length_class(std::max(n, m))];
+        const int widest = std::max(start, edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0));
+        for (int k = start; k <= widest; ++k) ++first[k];
+        for (int k = start + 1; k <= widest; ++k) ++retry[k];
+        for (int k = 0; k <= widest; ++k) ++trace[k];
+    }
+    for (int k = 0; k < EDIT_BINS; ++k) {
+        b->first_count[k] = first[k];
+        b->retry_reach[k] = retry[k];
+        b->trace_reach[k] = trace[k];
+    }
+    for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) b->seeded_start[c] = b->start_bin[c];
+    if (b->edit_graph) {
+        (void)hipGraphExecDestroy(b->edit_graph);
+        b->edit_graph = nullptr;
+    }
+}
+
+// Learned seeds without leaving the device (unit-cost batches): features, the network's predictions and each
+// pair's first band and queue, all on the batch's stream.
+int64_t seed_unit_learned(dinara_batch *b, dinara_mlp *net, double margin) {
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    if (slots == 0) return 0;
+    DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
+    DeviceGuard guard(ctx);
+    (void)hipGetLastError();
+    if (!b->features_d.reserve((size_t)slots * FEATURE_STRIDE * 4) || !b->predictions_d.reserve((size_t)slots * 4))
+        return DINARA_OUT_OF_MEMORY;
+    SweepArgs sa = sweep_args(b);
+    launch_features(sa.tape, (int)slots, b->reversed ? 1 : 0, static_cast<float *>(b->features_d.ptr), ctx->stream);
+    if (const int64_t code = mlp_predict_into(net, static_cast<const float *>(b->features_d.ptr), slots,
+                                              static_cast<float *>(b->predictions_d.ptr)))
+        return code;
+    EditStart start{};
+    for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) start.bin[c] = b->start_bin[c];
+    launch_edit_seed(edit_args(b), static_cast<const float *>(b->predictions_d.ptr), margin, start, (int)slots,
+                     ctx->stream);
+    seeded_shape(b);
+    return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
+}
+
 void drop_graph(dinara_batch *b) {
     if (b->edit_graph) {
         DeviceGuard guard(b->ctx);
@@ -1151,6 +1260,7 @@ int64_t autotune(dinara_batch *b, int64_t *chosen) {
         if (!chosen) return;
         chosen[0] = b->grid_per_cu;
         for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) chosen[1 + c] = b->start_bin[c];
+        chosen[1 + EDIT_LENGTH_CLASSES] = b->side_by_side_bins ? 1 : 0;
     };
     if (!b->unit || slots == 0) {
         report();
@@ -1186,6 +1296,19 @@ int64_t autotune(dinara_batch *b, int64_t *chosen) {
     }
     b->grid_per_cu = best_grid;
     drop_graph(b);
+    // The bins side by side or one after another.
+    b->side_by_side_bins = !b->side_by_side_bins;
+    drop_graph(b);
+    if (const double t = measure(); t < best)
+        best = t;
+    else
+        b->side_by_side_bins = !b->side_by_side_bins;
+    drop_graph(b);
+    // Queues the device filed from learned seeds keep their bands: the start bins stay.
+    if (b->seeded_start[0] >= 0) {
+        report();
+        return failure;
+    }
     // Length classes by weight (the sum of their pairs' squared longer sides), heaviest first.
     double weight[EDIT_LENGTH_CLASSES] = {};
     for (int64_t s = 0; s < slots; ++s) {
@@ -1224,6 +1347,7 @@ int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
 
 int64_t dinara_batch_seed_learned(dinara_batch *b, dinara_mlp *net, double margin) {
     if (!b || !net) return DINARA_INVALID_MODE;
+    if (b->unit) return seed_unit_learned(b, net, margin);
     const int64_t slots = (int64_t)b->slot_pair.size();
     if (slots == 0) return 0;
     std::vector<float> predicted((size_t)slots);
@@ -1256,6 +1380,25 @@ int64_t dinara_batch_set_retry(dinara_batch *b, int64_t doubling) {
 int64_t dinara_batch_set_graph(dinara_batch *b, int64_t on) {
     if (!b) return DINARA_INVALID_MODE;
     b->use_graph = on != 0;
+    return 0;
+}
+
+int64_t dinara_batch_set_keep_first(dinara_batch *b, int64_t on) {
+    if (!b) return DINARA_INVALID_MODE;
+    b->keep_first = on != 0;
+    drop_graph(b);
+    if (b->edit_arena.ptr) {  // its size follows the setting: made again at the next run
+        DeviceGuard guard(b->ctx);
+        b->edit_arena.release();
+        b->arena_capacity = 0;
+    }
+    return 0;
+}
+
+int64_t dinara_batch_set_side_by_side(dinara_batch *b, int64_t on) {
+    if (!b) return DINARA_INVALID_MODE;
+    b->side_by_side_bins = on != 0;
+    drop_graph(b);
     return 0;
 }
 
@@ -1305,6 +1448,7 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 6: return b->unit ? 1 : 0;
         case 7: return b->graph_used;
         case 8: return b->trace_passes;
+        case 9: return b->walked_first;
         default: break;
     }
     if (what >= 600 && what < 1000 && what % 100 < EDIT_BINS) {

@@ -18,6 +18,12 @@ struct DeviceContext {
     int compute_units = 0;
     std::vector<hipEvent_t> pool;  // timing events, made as needed
     hipEvent_t event(size_t i);
+    // Further streams for launches that may run side by side, forked from and joined back to `stream` by events;
+    // made as needed, default priority.
+    std::vector<hipStream_t> sides;
+    std::vector<hipEvent_t> marks;  // events that only order streams (no timing)
+    hipStream_t side(size_t i);
+    hipEvent_t mark(size_t i);
     std::mutex lock;  // one caller's work on the stream at a time (the events are shared)
 };
 
@@ -41,6 +47,7 @@ struct DeviceBuffer {
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;
     ~DeviceBuffer();
     bool reserve(size_t bytes);                                          // at least `bytes`, contents dropped
+    void release();                                                      // frees it now
     bool upload(DeviceContext *ctx, const void *host, size_t bytes);     // reserve, copy, wait
     bool write(DeviceContext *ctx, const void *host, size_t bytes);      // copy into what is there, wait
     bool read(DeviceContext *ctx, void *host, size_t bytes) const;       // copy out after the stream's work

@@ -57,6 +57,24 @@ hipEvent_t DeviceContext::event(size_t i) {
     return pool[i];
 }
 
+hipStream_t DeviceContext::side(size_t i) {
+    while (sides.size() <= i) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        sides.push_back(st);
+    }
+    return sides[i];
+}
+
+hipEvent_t DeviceContext::mark(size_t i) {
+    while (marks.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        marks.push_back(e);
+    }
+    return marks[i];
+}
+
 DeviceGuard::DeviceGuard(const DeviceContext *ctx) {
     hipGetDevice(&previous);
     if (ctx && ctx->id != previous) hipSetDevice(ctx->id);
@@ -68,6 +86,12 @@ DeviceGuard::~DeviceGuard() {
 
 DeviceBuffer::~DeviceBuffer() {
     if (ptr) (void)hipFree(ptr);
+}
+
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    size = 0;
 }
 
 bool DeviceBuffer::reserve(size_t bytes) {

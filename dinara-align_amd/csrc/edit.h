@@ -11,9 +11,10 @@
 //
 // Pairs are binned by band width. A wave holds 64 / lanes pairs, each on `lanes` adjacent lanes, and every pair
 // group takes its next pair from its bin's queue the moment it finishes the last one (a persistent launch): pairs of
-// very different lengths share waves without waiting for each other. A pair its band does not prove is filed into a
-// wider bin, which runs later in the same sequence of launches; that sequence has a fixed shape, so one hipGraph
-// replays it.
+// very different lengths share waves without waiting for each other. Every bin's first queue is swept at once, the
+// bins side by side on streams of their own (a long pair's tail in one bin overlaps the others' work); a pair its
+// band does not prove is filed into a retry queue of a wider bin, and the retry queues are swept after, narrowest
+// bin first. The launches have a fixed shape, so one hipGraph replays them.
 #pragma once
 
 #include "common.h"
@@ -32,10 +33,14 @@ struct EditArgs {
     int capacity;           // slots a list holds
     int32_t *band_low;      // per slot: the lowest diagonal of the band its next sweep takes
     const int32_t *first_band_low;  // per slot: its first band's, copied into `band_low` at the start of a run
-    int32_t *lists;         // EDIT_BINS sweep queues of `capacity` slots
+    int32_t *lists;         // the queues a launch sweeps: EDIT_BINS of `capacity` slots
     int32_t *counts;        // [0, EDIT_BINS) queue lengths, [EDIT_BINS, 2 EDIT_BINS) queue heads
+    int32_t *file_lists;    // the queues a launch files unproven pairs into (laid out as `lists`)
+    int32_t *file_counts;
     int32_t *first_lists;   // the first sweep's queues, copied into `lists` at the start of a run
     const int32_t *first_counts;
+    int32_t *first_counts_out;  // the same counts, written by the seeding kernel
+    int32_t *first_low_out;     // the first bands' lowest diagonals, written by the seeding kernel
     // alignments: the band each settled pair is traced in, its decisions' place in the arena, and the queues of the
     // tracing sweep and of the walk
     int32_t *trace_lists;   // EDIT_BINS queues of `capacity` slots
@@ -48,9 +53,17 @@ struct EditArgs {
     uint8_t *arena;
     unsigned long long *arena_used;
     unsigned long long arena_capacity;
-    uint32_t *stats;        // sweeps begun by bin, then trace sweeps by bin, then pairs their first band left unproven
+    uint32_t *stats;        // sweeps begun by bin, then trace sweeps by bin, then pairs their first band left unproven,
+                            // then pairs proven with their decisions kept
     int one_pair;           // 1: each pair group sweeps one pair and stops (no refill), for comparison
+    int keep_first;         // 1 (alignments): every sweep keeps its decisions, and a pair it proves walks them
 };
+
+// The bytes a pair of `cols` query letters traced in bin `bin` keeps: its columns eight at a time, each word's eight
+// columns 128 contiguous bytes (two 64-bit planes a column), the pair's words side by side.
+__host__ __device__ inline unsigned long long edit_trace_bytes(int cols, int bin) {
+    return (unsigned long long)((cols + 7) / 8) * edit_lanes(bin) * edit_words(bin) * 128ull;
+}
 
 // Pairs' longer sides split at 256, 1024, 4096 and 16384 letters: each class has its own least first bin, the
 // tuned start of its band doubling.
@@ -77,10 +90,18 @@ __host__ __device__ inline int edit_place(int want_low, int want_high, int width
     return low;
 }
 
+// Each length class's least first bin.
+struct EditStart {
+    int bin[EDIT_LENGTH_CLASSES];
+};
+
 // Host-side entry points of edit.hip. `grid` is the workgroups of each persistent launch (256 threads each).
 void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream);
 void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream);
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream);
+// Every slot's first band and first queue from predictions of log(1 + cost) in device memory (learned seeds).
+void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, const EditStart &start, int slots,
+                      hipStream_t stream);
 // Empties the arena and the trace and walk queues, then queues `count` deferred pairs (`held`, a copy of the
 // deferred list) for a further tracing pass.
 void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream);

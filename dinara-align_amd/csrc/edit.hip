@@ -92,6 +92,12 @@ struct Cursor {
     }
 };
 
+// Where the four columns 4q..4q+3 of word w of a pair's `words` words lie in its kept decisions: eight columns of a
+// word take 128 contiguous bytes (two 64-bit planes a column), the words of an eight-column block side by side.
+__device__ __forceinline__ size_t trace_chunk(int q, int words, int w) {
+    return ((size_t)(q >> 1) * words + w) * 128 + (size_t)(q & 1) * 64;
+}
+
 // ------------------------------------------------------------------ settling and filing (first lanes only)
 
 __device__ void settle(const EditArgs &a, int slot, int n, int m, int64_t found) {
@@ -115,7 +121,7 @@ __device__ void settle(const EditArgs &a, int slot, int n, int m, int64_t found)
     const int low = edit_place(l, h, edit_width(bin), n, m);
     s.walk_low[slot] = low;
     s.walk_class[slot] = bin;
-    const unsigned long long bytes = (unsigned long long)m * edit_lanes(bin) * edit_words(bin) * 16ull;
+    const unsigned long long bytes = edit_trace_bytes(m, bin);
     const unsigned long long at = atomicAdd(a.arena_used, bytes);
     if (at + bytes > a.arena_capacity) {
         a.deferred[atomicAdd(a.deferred_count, 1)] = slot;
@@ -141,17 +147,24 @@ __device__ void refile(const EditArgs &a, int slot, int n, int m, int lo, int hi
     }
     a.band_low[slot] = edit_place(want_low, want_high, edit_width(next), n, m);
     a.s.status[slot] = 1;
-    a.lists[(size_t)next * a.capacity + atomicAdd(&a.counts[next], 1)] = slot;
+    a.file_lists[(size_t)next * a.capacity + atomicAdd(&a.file_counts[next], 1)] = slot;
 }
 
 // ------------------------------------------------------------------ the sweep
 
-// Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. TRACE: the second pass of an
-// alignment batch, which sweeps each settled pair's traced band again and keeps a 2-bit decision per cell (0 two
-// unequal letters, 1 a reference letter alone, 2 a query letter alone, 3 two equal letters: the walk's first true
-// choice in that order) in two bit-planes, plane-major per word: word w's column c at arena + offset + (w m + c) 16.
-template <int L, int K, int BITS, bool TRACE>
+// The sweep's three uses: a pass that proves bands (SWEEP), the second pass of an alignment batch that sweeps each
+// settled pair's traced band again keeping its decisions (TRACE), and a pass that proves bands and keeps the
+// decisions of each band as it goes (SWEEP_TRACED), so a pair its first band proves goes straight to the walk.
+constexpr int SWEEP = 0, TRACE = 1, SWEEP_TRACED = 2;
+
+// Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. Kept decisions are 2 bits a cell (0
+// two unequal letters, 1 a reference letter alone, 2 a query letter alone, 3 two equal letters: the walk's first
+// true choice in that order) in two bit-planes: word w's columns 8q..8q+7 in the 128 bytes at arena + offset +
+// (q words + w) 128, 16 a column. Each lane holds its last four columns and writes their 64 bytes at once.
+template <int L, int K, int BITS, int MODE>
 __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
+    constexpr bool TRACE = MODE == 1;          // sweeps settled pairs' traced bands again
+    constexpr bool RECORD = MODE != 0;         // keeps decisions
     constexpr int W = 64 * L * K;
     const int lane = threadIdx.x & 63;
     const int k = lane & (L - 1);
@@ -169,6 +182,14 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     bottom.begin(nullptr, 0, 0);
     query.begin(nullptr, 0, 0);
     uint8_t *trace = nullptr;
+    bool recording = false;    // this group keeps the current pair's decisions
+    long long trace_at = 0;    // and where, from the arena's start
+    constexpr int WL = L * K;  // the pair's words
+    uint64_t held0[4][K], held1[4][K];  // the last four columns' decision planes, oldest first
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) held0[q][kk] = held1[q][kk] = 0;
 #pragma unroll
     for (int kk = 0; kk < K; ++kk) {
         pv[kk] = mv[kk] = 0;
@@ -215,7 +236,23 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     }
                     bottom.begin(ref, n, -lo);
                     query.begin(qry, m, 0);
-                    if (TRACE) trace = a.arena + a.trace_offset[slot] + (size_t)(k * K) * m * 16;
+                    if (TRACE) {
+                        trace_at = a.trace_offset[slot];
+                        recording = true;
+                    } else if (RECORD) {
+                        // Room for this band's decisions, if the arena has it; else the pair sweeps without.
+                        long long at = -1;
+                        if (first_lane && m > 0) {
+                            const unsigned long long bytes = edit_trace_bytes(m, bin);
+                            const unsigned long long got = atomicAdd(a.arena_used, bytes);
+                            if (got + bytes <= a.arena_capacity) at = (long long)got;
+                        }
+                        const int at_lo = __shfl((int)(at & 0xFFFFFFFF), lane - k);
+                        const int at_hi = __shfl((int)(at >> 32), lane - k);
+                        trace_at = ((long long)at_hi << 32) | (long long)(unsigned)at_lo;
+                        recording = trace_at >= 0;
+                    }
+                    trace = recording ? a.arena + trace_at : a.arena;
                     if (first_lane) atomicAdd(&a.stats[(TRACE ? EDIT_BINS : 0) + bin], 1u);
                 } else {
                     drained = true;
@@ -242,6 +279,28 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 #pragma unroll
             for (int d = L / 2; d >= 1; d /= 2) part += __shfl_xor(part, d);
             const int found = hi + m + top + part;  // `top` is the first lane's
+            if constexpr (RECORD) {
+                // The last, partly filled four columns: shifted down to the chunk's start, then written whole.
+                if (finishing && recording && (m & 3)) {
+                    for (int pad = m & 3; pad < 4; ++pad) {
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) {
+                            held0[0][kk] = held0[1][kk];
+                            held0[1][kk] = held0[2][kk];
+                            held0[2][kk] = held0[3][kk];
+                            held1[0][kk] = held1[1][kk];
+                            held1[1][kk] = held1[2][kk];
+                            held1[2][kk] = held1[3][kk];
+                        }
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) {
+                        ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + trace_chunk(m >> 2, WL, k * K + kk));
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
+                    }
+                }
+            }
             if (first_lane && finishing) {
                 if (TRACE) {
                     // The traced band was proven in the first pass: its cost must agree.
@@ -249,10 +308,20 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
                 } else {
                     int want_low = 0, want_high = 0;
-                    if (certify(a.s, n, m, found, lo, hi, &want_low, &want_high))
-                        settle(a, slot, n, m, found);
-                    else
+                    if (!certify(a.s, n, m, found, lo, hi, &want_low, &want_high)) {
                         refile(a, slot, n, m, lo, hi, want_low, want_high, bin);
+                    } else if (RECORD && recording && found <= a.s.cap) {
+                        // Proven with its decisions kept: walk this band.
+                        a.s.cost[slot] = (int32_t)(found * a.scale);
+                        a.s.status[slot] = 0;
+                        a.s.walk_low[slot] = lo;
+                        a.s.walk_class[slot] = bin;
+                        a.trace_offset[slot] = trace_at;
+                        a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+                        atomicAdd(&a.stats[2 * EDIT_BINS + 1], 1u);
+                    } else {
+                        settle(a, slot, n, m, found);
+                    }
                 }
             }
             if (finishing) slot = -1;
@@ -342,14 +411,21 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             const uint64_t mhs = (mh[kk] << 1) | (kk ? mh[kk > 0 ? kk - 1 : 0] >> 63 : (uint64_t)((carried >> 1) & 1));
             const uint64_t npv = mhs | ~(xv[kk] | phs);
             const uint64_t nmv = phs & xv[kk];
-            if constexpr (TRACE) {
+            if constexpr (RECORD) {
                 // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
                 const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
-                const uint64_t plane0 = ~unequal_step & (npv | ~ph[kk]);
-                const uint64_t plane1 = ~unequal_step & ~npv;
-                if (active) {
-                    ulonglong2 *cell = reinterpret_cast<ulonglong2 *>(trace + ((size_t)kk * m + (j - 1)) * 16);
-                    *cell = make_ulonglong2(plane0, plane1);
+                held0[0][kk] = held0[1][kk];
+                held0[1][kk] = held0[2][kk];
+                held0[2][kk] = held0[3][kk];
+                held0[3][kk] = ~unequal_step & (npv | ~ph[kk]);
+                held1[0][kk] = held1[1][kk];
+                held1[1][kk] = held1[2][kk];
+                held1[2][kk] = held1[3][kk];
+                held1[3][kk] = ~unequal_step & ~npv;
+                if (active && recording && (j & 3) == 0) {
+                    ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + trace_chunk((j >> 2) - 1, WL, k * K + kk));
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
                 }
             }
             pv[kk] = npv;
@@ -377,16 +453,37 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
         OpTape ops = op_tape(a.s, slot);
         const uint32_t unequal = a.s.eqx ? 1u : 0u;
         const int width = edit_width(bin);
+        const int words = edit_lanes(bin) * edit_words(bin);
         int r = n, c = m;
+        // The 128 bytes held: eight columns of one word (as eight-column block times words + word), plane 0 of
+        // column k in h[k], plane 1 in g[k].
+        int held_chunk = -1;
+        uint64_t h[8] = {}, g[8] = {};
         while (r > 0 && c > 0) {
             const int i = hi - c + r;
             if (i < 0 || i >= width) {  // a proven band holds every step; this pair's did not
                 a.s.status[slot] = 3;
                 break;
             }
-            const uint64_t *cell = reinterpret_cast<const uint64_t *>(base + ((size_t)(i >> 6) * m + (c - 1)) * 16);
+            const int want = ((c - 1) >> 3) * words + (i >> 6);
+            if (want != held_chunk) {
+                const uint64_t *at = reinterpret_cast<const uint64_t *>(base + (size_t)want * 128);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    h[k] = at[2 * k];
+                    g[k] = at[2 * k + 1];
+                }
+                held_chunk = want;
+            }
+            const int pos = (c - 1) & 7;
+            uint64_t plane0 = h[0], plane1 = g[0];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) {
+                plane0 = pos == k ? h[k] : plane0;
+                plane1 = pos == k ? g[k] : plane1;
+            }
             const uint32_t bit = i & 63;
-            const uint32_t code = (uint32_t)((cell[0] >> bit) & 1) | ((uint32_t)((cell[1] >> bit) & 1) << 1);
+            const uint32_t code = (uint32_t)((plane0 >> bit) & 1) | ((uint32_t)((plane1 >> bit) & 1) << 1);
             if (code == 1) {
                 ops.put(2u);
                 --r;
@@ -422,10 +519,12 @@ __global__ void edit_reset(EditArgs a, int slots) {
     if (t < EDIT_BINS) {
         a.counts[t] = a.first_counts[t];
         a.counts[EDIT_BINS + t] = 0;
+        a.file_counts[t] = 0;
+        a.file_counts[EDIT_BINS + t] = 0;
         a.trace_counts[t] = 0;
         a.trace_counts[EDIT_BINS + t] = 0;
     }
-    if (t <= 2 * EDIT_BINS) a.stats[t] = 0;
+    if (t <= 2 * EDIT_BINS + 1) a.stats[t] = 0;
     if (t == 0) {
         *a.walk_count = 0;
         *a.deferred_count = 0;
@@ -452,8 +551,7 @@ __global__ void edit_requeue(EditArgs a, const int32_t *held, int count) {
     if (t >= count) return;
     const int slot = held[t];
     const int bin = a.s.walk_class[slot];
-    const unsigned long long bytes =
-        (unsigned long long)a.s.tape.cols[slot] * edit_lanes(bin) * edit_words(bin) * 16ull;
+    const unsigned long long bytes = edit_trace_bytes(a.s.tape.cols[slot], bin);
     const unsigned long long at = atomicAdd(a.arena_used, bytes);
     if (at + bytes > a.arena_capacity) {
         a.deferred[atomicAdd(a.deferred_count, 1)] = slot;
@@ -463,41 +561,98 @@ __global__ void edit_requeue(EditArgs a, const int32_t *held, int count) {
     a.trace_lists[(size_t)bin * a.capacity + atomicAdd(&a.trace_counts[bin], 1)] = slot;
 }
 
-template <int L, int K, bool TRACE>
-void sweep_bits(const EditArgs &a, int bin, int grid, hipStream_t stream) {
-    if (a.s.tape.bits == 2)
-        hipLaunchKernelGGL((edit_sweep<L, K, 2, TRACE>), dim3(grid), dim3(256), 0, stream, a, bin);
-    else
-        hipLaunchKernelGGL((edit_sweep<L, K, 8, TRACE>), dim3(grid), dim3(256), 0, stream, a, bin);
+// Each slot's first band from the network's prediction of its cost (log(1 + cost)): the guess e^(prediction +
+// margin) - 1, and the band out from the start and end diagonals to the last diagonal a path of the guess's cost
+// could visit (unit costs: a path through diagonal k > max(0, end) costs at least 2k - end, through k < min(0, end)
+// at least end - 2k), one more either side; then the slot is filed in its bin's first queue. A workgroup's slots
+// (consecutive, so of like length) take one place in each queue with one atomic a bin.
+__global__ void edit_seed(EditArgs a, const float *prediction, double margin, EditStart start, int slots) {
+    __shared__ int local[EDIT_BINS], base[EDIT_BINS];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (threadIdx.x < EDIT_BINS) local[threadIdx.x] = 0;
+    __syncthreads();
+    int bin = -1, rank = 0, low = 0;
+    if (t < slots) {
+        const int n = a.s.tape.rows[t], m = a.s.tape.cols[t], end = m - n;
+        const double guessed = expm1((double)prediction[t] + margin);
+        const long long g = guessed > 0 ? (long long)guessed : 0;
+        const long long from_up = end > 0 ? end : 0, from_down = end < 0 ? end : 0;
+        long long up = (g + end) >= 0 ? (g + end) / 2 : -((-(g + end) + 1) / 2);  // floor((g + end) / 2)
+        long long down = (end - g) >= 0 ? (end - g + 1) / 2 : -((g - end) / 2);   // ceil((end - g) / 2)
+        up = up > from_up ? up : from_up;
+        up = up < m + 1 ? up : m + 1;
+        down = down < from_down ? down : from_down;
+        down = down > -(n + 1) ? down : -(n + 1);
+        long long hi = from_up + 1 > up + 1 ? from_up + 1 : up + 1;
+        long long lo = from_down - 1 < down - 1 ? from_down - 1 : down - 1;
+        hi = hi < m + 1 ? hi : m + 1;
+        lo = lo > -n - 1 ? lo : -n - 1;
+        const int longer = n > m ? n : m;
+        bin = edit_bin_for((int)(hi - lo + 1), start.bin[edit_length_class(longer)]);
+        if (bin >= 0) {
+            low = edit_place((int)lo, (int)hi, edit_width(bin), n, m);
+            rank = atomicAdd(&local[bin], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < EDIT_BINS)
+        base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(&a.first_counts_out[threadIdx.x], local[threadIdx.x]) : 0;
+    __syncthreads();
+    if (t < slots) {
+        a.first_low_out[t] = low;
+        if (bin >= 0) a.first_lists[(size_t)bin * a.capacity + base[bin] + rank] = t;
+    }
 }
 
-template <bool TRACE>
+__global__ void edit_seed_clear(int32_t *counts) {
+    if (threadIdx.x < EDIT_BINS) counts[threadIdx.x] = 0;
+}
+
+template <int L, int K, int MODE>
+void sweep_bits(const EditArgs &a, int bin, int grid, hipStream_t stream) {
+    if (a.s.tape.bits == 2)
+        hipLaunchKernelGGL((edit_sweep<L, K, 2, MODE>), dim3(grid), dim3(256), 0, stream, a, bin);
+    else
+        hipLaunchKernelGGL((edit_sweep<L, K, 8, MODE>), dim3(grid), dim3(256), 0, stream, a, bin);
+}
+
+template <int MODE>
 void sweep_bin(const EditArgs &a, int bin, int grid, hipStream_t stream) {
     switch (bin) {
-        case 0: sweep_bits<1, 1, TRACE>(a, bin, grid, stream); break;
-        case 1: sweep_bits<2, 1, TRACE>(a, bin, grid, stream); break;
-        case 2: sweep_bits<4, 1, TRACE>(a, bin, grid, stream); break;
-        case 3: sweep_bits<8, 1, TRACE>(a, bin, grid, stream); break;
-        case 4: sweep_bits<16, 1, TRACE>(a, bin, grid, stream); break;
-        case 5: sweep_bits<32, 1, TRACE>(a, bin, grid, stream); break;
-        case 6: sweep_bits<64, 1, TRACE>(a, bin, grid, stream); break;
-        case 7: sweep_bits<64, 2, TRACE>(a, bin, grid, stream); break;
-        default: sweep_bits<64, 4, TRACE>(a, bin, grid, stream); break;
+        case 0: sweep_bits<1, 1, MODE>(a, bin, grid, stream); break;
+        case 1: sweep_bits<2, 1, MODE>(a, bin, grid, stream); break;
+        case 2: sweep_bits<4, 1, MODE>(a, bin, grid, stream); break;
+        case 3: sweep_bits<8, 1, MODE>(a, bin, grid, stream); break;
+        case 4: sweep_bits<16, 1, MODE>(a, bin, grid, stream); break;
+        case 5: sweep_bits<32, 1, MODE>(a, bin, grid, stream); break;
+        case 6: sweep_bits<64, 1, MODE>(a, bin, grid, stream); break;
+        case 7: sweep_bits<64, 2, MODE>(a, bin, grid, stream); break;
+        default: sweep_bits<64, 4, MODE>(a, bin, grid, stream); break;
     }
 }
 
 }  // namespace
 
 void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream) {
-    const int threads = slots > 2 * EDIT_BINS + 1 ? slots : 2 * EDIT_BINS + 1;
+    const int threads = slots > 2 * EDIT_BINS + 2 ? slots : 2 * EDIT_BINS + 2;
     hipLaunchKernelGGL(edit_reset, dim3((threads + 255) / 256), dim3(256), 0, stream, a, slots);
 }
 
 void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream) {
     if (trace)
-        sweep_bin<true>(a, bin, grid, stream);
+        sweep_bin<TRACE>(a, bin, grid, stream);
+    else if (a.keep_first)
+        sweep_bin<SWEEP_TRACED>(a, bin, grid, stream);
     else
-        sweep_bin<false>(a, bin, grid, stream);
+        sweep_bin<SWEEP>(a, bin, grid, stream);
+}
+
+void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, const EditStart &start, int slots,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL(edit_seed_clear, dim3(1), dim3(64), 0, stream, a.first_counts_out);
+    if (slots > 0)
+        hipLaunchKernelGGL(edit_seed, dim3((slots + 255) / 256), dim3(256), 0, stream, a, prediction, margin, start,
+                           slots);
 }
 
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {

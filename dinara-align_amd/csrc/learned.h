@@ -23,3 +23,6 @@ struct dinara_mlp;
 // The network's predictions (log(1 + cost)) for `rows` rows of fp32 features already in device memory, into `out`
 // (host memory). 0, or a DINARA_* code.
 int64_t mlp_predict_device(dinara_mlp *net, const float *features, int64_t rows, float *out);
+
+// The same into `out` in device memory, ordered on the network's stream with no wait for the host.
+int64_t mlp_predict_into(dinara_mlp *net, const float *features, int64_t rows, float *out);

@@ -202,6 +202,8 @@ struct dinara_mlp {
     DeviceBuffer master, bf, grad, m1, m2;
     // per-step work
     DeviceBuffer x, z1, a1, z2, a2, y, dy, sq, dz2, dz1, targets, index, loss, features;
+    DeviceBuffer identity;  // 0, 1, 2, ... for gathers of consecutive rows
+    int identity_rows = 0;
     size_t off_w1, off_b1, off_w2, off_b2, off_w3, off_b3, total;
 };
 
@@ -304,6 +306,34 @@ int64_t mlp_predict_device(dinara_mlp *net, const float *features, int64_t rows,
         if (!net->y.read(net->ctx, out + first, (size_t)count * 4)) return DINARA_OUT_OF_MEMORY;
     }
     return 0;
+}
+
+int64_t mlp_predict_into(dinara_mlp *net, const float *features, int64_t rows, float *out) {
+    if (!net) return DINARA_INVALID_MODE;
+    if (rows <= 0) return 0;
+    DeviceGuard guard(net->ctx);
+    hipStream_t s = net->ctx->stream;
+    const int chunk = 1 << 16;
+    if (!reserve_rows(net, chunk)) return DINARA_OUT_OF_MEMORY;
+    if (net->identity_rows < chunk) {
+        std::vector<int32_t> idx((size_t)chunk);
+        for (int i = 0; i < chunk; ++i) idx[i] = i;
+        if (!net->identity.upload(net->ctx, idx.data(), idx.size() * 4)) return DINARA_OUT_OF_MEMORY;
+        net->identity_rows = chunk;
+    }
+    for (int64_t first = 0; first < rows; first += chunk) {
+        const int count = (int)std::min<int64_t>(chunk, rows - first);
+        const int padded = (count + 31) / 32 * 32;
+        hipLaunchKernelGGL(gather_rows, dim3(padded), dim3(FEATURE_STRIDE), 0, s, features + first * FEATURE_STRIDE,
+                           static_cast<const int32_t *>(net->identity.ptr), count, h16(net->x), padded);
+        forward(net, padded, s);
+        hipLaunchKernelGGL(head, dim3((padded + 255) / 256), dim3(256), 0, s, h16(net->a2), h16(net->bf, net->off_w3),
+                           f32(net->master, net->off_b3), net->hidden, padded, count, nullptr, f32(net->y),
+                           nullptr, nullptr, nullptr, nullptr);
+        if (hipMemcpyAsync(out + first, net->y.ptr, (size_t)count * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return DINARA_OUT_OF_MEMORY;
+    }
+    return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
 }
 
 extern "C" {

@@ -127,6 +127,10 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_set_retry.restype = i64
     handle.dinara_batch_set_graph.argtypes = [vp, i64]
     handle.dinara_batch_set_graph.restype = i64
+    handle.dinara_batch_set_keep_first.argtypes = [vp, i64]
+    handle.dinara_batch_set_keep_first.restype = i64
+    handle.dinara_batch_set_side_by_side.argtypes = [vp, i64]
+    handle.dinara_batch_set_side_by_side.restype = i64
     handle.dinara_batch_set_persistent.argtypes = [vp, i64]
     handle.dinara_batch_set_persistent.restype = i64
     handle.dinara_batch_set_tuning.argtypes = [vp, i64, vp]

@@ -204,7 +204,7 @@ void dinara_batch_free(dinara_batch *batch);
  * pair under a user band), 10-21 one pair a block of waves (512, 1024, 1536, 2048, 3072, 4096..8192 in steps of
  * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
- * needed; for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
+ * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep); for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
  * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began. For roofline
  * reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
@@ -249,14 +249,23 @@ int64_t dinara_batch_set_retry(dinara_batch *batch, int64_t doubling);
 /* Runs replay one captured hipGraph of the run's launches (1, the default) or make them one by one with an event
  * pair around each, for per-kernel times (0). */
 int64_t dinara_batch_set_graph(dinara_batch *batch, int64_t on);
+/* Alignments: every sweep keeps its decisions and a pair its band proves walks them at once (1), or the sweeps
+ * only prove bands and each settled pair's narrowest proven band is swept again keeping them (0, the default).
+ * With good first bands (learned seeds) the first saves the second sweep; with poor ones it keeps decisions of
+ * bands that fail. */
+int64_t dinara_batch_set_keep_first(dinara_batch *batch, int64_t on);
 /* Lane groups refill from their bin's queue (1, the default) or sweep one pair each (0: a launch as wide as the
  * bin's pairs, for comparison). */
 int64_t dinara_batch_set_persistent(dinara_batch *batch, int64_t on);
 /* The launch shape: persistent workgroups a compute unit (1..64), and for each class of the pairs' longer side
  * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..8); null keeps them. */
 int64_t dinara_batch_set_tuning(dinara_batch *batch, int64_t grid_per_cu, const int64_t *start_bins);
-/* Chooses that shape by timing the batch's own first pass (workgroups a compute unit, then each class's start bin)
- * and keeps it; writes the choice to `chosen` (1 + DINARA_EDIT_LENGTH_CLASSES values) unless null. */
+/* Each bin's first queue, and each bin's tracing sweep, on a stream of its own so the bins run side by side (1, the
+ * default), or the bins one after another (0). */
+int64_t dinara_batch_set_side_by_side(dinara_batch *batch, int64_t on);
+/* Chooses that shape by timing the batch's own first pass (workgroups a compute unit, side by side or not, then
+ * each class's start bin, which stays as it is after learned seeding) and keeps it; writes the choice to `chosen`
+ * (workgroups, the DINARA_EDIT_LENGTH_CLASSES start bins, side by side) unless null. */
 int64_t dinara_batch_autotune(dinara_batch *batch, int64_t *chosen);
 
 dinara_mlp *dinara_mlp_new(int64_t gpu, uint64_t seed);

@@ -220,10 +220,49 @@ def test_launch_shapes_change_no_result():
         "one pair a group": lambda b: h.dinara_batch_set_persistent(b.ptr, 0),
         "doubling": lambda b: h.dinara_batch_set_retry(b.ptr, 1),
         "wide start": lambda b: h.dinara_batch_set_tuning(b.ptr, 2, (ctypes.c_int64 * 5)(3, 3, 4, 5, 6)),
+        "keep first": lambda b: h.dinara_batch_set_keep_first(b.ptr, 1),
+        "keep first, doubling, one pair a group": lambda b: (h.dinara_batch_set_keep_first(b.ptr, 1),
+                                                            h.dinara_batch_set_retry(b.ptr, 1),
+                                                            h.dinara_batch_set_persistent(b.ptr, 0)),
     }
     for name, setup in shapes.items():
         got, _ = results_with(refs, qrys, 1, setup)
         assert got == want, name
+
+
+def test_keep_first_walks_proven_bands_at_once():
+    refs, qrys = uneven(300, 67)
+    want = oracle_alignments(refs, qrys)
+    h = _lib.lib()
+    b = Batch(refs, qrys, 1)
+    try:
+        h.dinara_batch_set_keep_first(b.ptr, 1)
+        got = b.run()
+        walked = b.stat(9)
+        # Seeds at each pair's true cost: every first band proves its pair and is walked as swept.
+        seeds = np.ascontiguousarray([c for c, _ in want], dtype=np.int64)
+        assert h.dinara_batch_seed(b.ptr, seeds.ctypes.data) == 0
+        seeded = b.run()
+        seeded_walked = b.stat(9)
+    finally:
+        b.free()
+    assert got == want and seeded == want
+    assert 0 < walked <= len(refs)
+    assert seeded_walked == len(refs)
+
+
+def test_keep_first_with_a_small_arena(monkeypatch):
+    # Bands that find no room sweep without keeping decisions and are traced in a second sweep.
+    refs, qrys = uneven(200, 71)
+    monkeypatch.setenv("DINARA_TRACE_ARENA", str(1 << 20))
+    h = _lib.lib()
+    b = Batch(refs, qrys, 1)
+    try:
+        h.dinara_batch_set_keep_first(b.ptr, 1)
+        got = b.run()
+    finally:
+        b.free()
+    assert got == oracle_alignments(refs, qrys)
 
 
 def test_small_arena_traces_in_several_passes(monkeypatch):
@@ -246,7 +285,7 @@ def test_autotune_keeps_every_result():
     h = _lib.lib()
     b = Batch(refs, qrys, 1)
     try:
-        chosen = (ctypes.c_int64 * 6)()
+        chosen = (ctypes.c_int64 * 7)()
         assert h.dinara_batch_autotune(b.ptr, chosen) == 0
         assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < 9 for i in range(1, 6))
         got = b.run()
@@ -268,3 +307,37 @@ def test_seeds_change_no_result():
             assert b.run() == want
         finally:
             b.free()
+
+
+def test_learned_seeds_on_the_device_change_no_result():
+    # An untrained network (poor seeds) and one trained briefly on the aligner's own distances: the device files
+    # every pair's first band from the prediction, and every result still equals the oracle's.
+    refs, qrys = uneven(400, 73)
+    want = oracle_alignments(refs, qrys)
+    h = _lib.lib()
+    train_r, train_q = uneven(3000, 79)
+    t = Batch(train_r, train_q, 0)
+    try:
+        x = np.zeros((len(train_r), 288), dtype=np.float32)
+        assert h.dinara_batch_features(t.ptr, x.ctypes.data) == 0
+        labels = np.log1p(np.array(t.run(), dtype=np.float64)).astype(np.float32)
+    finally:
+        t.free()
+    for trained in (False, True):
+        net = h.dinara_mlp_new(0, 9)
+        try:
+            if trained:
+                loss = ctypes.c_double(0)
+                assert h.dinara_mlp_train(net, len(labels), x.ctypes.data, labels.ctypes.data, 8, 256, 2e-3, 7,
+                                          ctypes.byref(loss)) == 0
+            for keep_first in (0, 1):
+                b = Batch(refs, qrys, 1)
+                try:
+                    h.dinara_batch_set_keep_first(b.ptr, keep_first)
+                    assert h.dinara_batch_seed_learned(b.ptr, net, ctypes.c_double(0.05)) == 0
+                    assert b.run() == want, (trained, keep_first)
+                    assert b.run() == want  # the replayed graph reads the same device-filed queues
+                finally:
+                    b.free()
+        finally:
+            h.dinara_mlp_free(net)
