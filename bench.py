@@ -571,7 +571,8 @@ def bin_profile(handle, b):
     if handle.dinara_batch_run(b) != 0:
         raise RuntimeError("profiling run failed")
     prof = {k: [handle.dinara_batch_stat(b, base + i) for i in range(EDIT_BINS)]
-            for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900))}
+            for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900),
+                            ("sweep_word_columns", 1000), ("trace_word_columns", 1100))}
     prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
     prof["run_ms"] = handle.dinara_batch_stat(b, 0)
     handle.dinara_batch_set_graph(b, 1)

@@ -209,11 +209,12 @@ struct dinara_batch {
     int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
     DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
         trace_counts, trace_offset, walk_list, walk_count, deferred, deferred_count, deferred_held, edit_arena,
-        arena_used, edit_stats;
+        arena_used, edit_stats, edit_work;
     size_t arena_capacity = 0;
     hipGraphExec_t edit_graph = nullptr;
     double edit_ms[EDIT_BINS] = {}, trace_ms[EDIT_BINS] = {}, edit_sweeps[EDIT_BINS] = {},
            trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0, walked_first = 0;
+    double sweep_work[EDIT_BINS] = {}, trace_work[EDIT_BINS] = {};  // word-columns swept by each bin
 };
 
 namespace {
@@ -537,7 +538,7 @@ int64_t unit_lists(dinara_batch *b) {
         lists[bin * cap + counts[bin]++] = (int32_t)s;
         // A band placed inside the matrix's diagonals and n + m + 3 wide proves any pair, and no traced band is
         // wider than n + m + 1: the bins a pair can reach.
-        const int widest = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
+        const int widest = edit_widest(n + m);
         ++first_count[bin];
         for (int k = bin + 1; k <= widest; ++k) ++retry_reach[k];
         for (int k = 0; k <= widest; ++k) ++trace_reach[k];
@@ -559,7 +560,7 @@ int64_t unit_lists(dinara_batch *b) {
     DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
                             &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
                             &b->walk_list, &b->walk_count, &b->deferred, &b->deferred_count, &b->deferred_held,
-                            &b->arena_used, &b->edit_stats};
+                            &b->arena_used, &b->edit_stats, &b->edit_work};
     void *before[sizeof(held) / sizeof(held[0])];
     for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i) before[i] = held[i]->ptr;
     const int64_t code = unit_buffers(b, lists, counts, cap);
@@ -582,7 +583,8 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_offset.reserve(cap * 8) || !b->walk_list.reserve(cap * 4) || !b->walk_count.reserve(4) ||
         !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
-        !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4))
+        !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4) ||
+        !b->edit_work.reserve(2 * EDIT_BINS * 8))
         return DINARA_OUT_OF_MEMORY;
     return 0;
 }
@@ -616,6 +618,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.arena_used = static_cast<unsigned long long *>(b->arena_used.ptr);
     e.arena_capacity = b->arena_capacity;
     e.stats = static_cast<uint32_t *>(b->edit_stats.ptr);
+    e.work = static_cast<unsigned long long *>(b->edit_work.ptr);
     e.one_pair = b->persistent ? 0 : 1;
     e.keep_first = b->keep_first && b->kind == DINARA_BATCH_ALIGNMENTS ? 1 : 0;
     return e;
@@ -637,7 +640,7 @@ size_t arena_bytes(const dinara_batch *b) {
     size_t bound = 0;
     for (size_t s = 0; s < b->slot_pair.size(); ++s) {
         const int n = b->rows[s], m = b->cols[s];
-        const int bin = edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0);
+        const int bin = edit_widest(n + m);
         bound += (size_t)edit_trace_bytes(m, bin);
     }
     size_t free_bytes = 0, total = 0;
@@ -653,11 +656,13 @@ constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, 
 
 // Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
 // stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
-// EV + 2 bin + 1 when `timed`.
+// EV + 2 bin + 1 when `timed`. Side by side, the widest bin is launched first: its long pairs are the run's
+// longest chains, and the hardware queues take the streams in order.
 template <typename Use, typename Launch>
 void side_by_side(DeviceContext *ctx, int mark_base, bool timed, int ev, bool apart, Use use, Launch launch) {
     (void)hipEventRecord(ctx->mark(mark_base), ctx->stream);
-    for (int bin = 0; bin < EDIT_BINS; ++bin) {
+    for (int k = 0; k < EDIT_BINS; ++k) {
+        const int bin = apart ? EDIT_BINS - 1 - k : k;
         if (!use(bin)) continue;
         if (!apart) {  // one after another on the main stream
             if (timed) (void)hipEventRecord(ctx->event(ev + 2 * bin), ctx->stream);
@@ -785,10 +790,14 @@ int64_t run_unit(dinara_batch *b, bool trace) {
     b->run_ms = total;
     b->sweep_ms = sweeps;
     uint32_t stats[2 * EDIT_BINS + 2] = {};
-    if (!b->edit_stats.read(ctx, stats, sizeof(stats))) return DINARA_OUT_OF_MEMORY;
+    unsigned long long work[2 * EDIT_BINS] = {};
+    if (!b->edit_stats.read(ctx, stats, sizeof(stats)) || !b->edit_work.read(ctx, work, sizeof(work)))
+        return DINARA_OUT_OF_MEMORY;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
         b->edit_sweeps[bin] = stats[bin];
         b->trace_sweeps[bin] = stats[EDIT_BINS + bin];
+        b->sweep_work[bin] = (double)work[bin];
+        b->trace_work[bin] = (double)work[EDIT_BINS + bin];
         if (!graph) {
             float ms = 0;
             if (b->first_count[bin] > 0 &&
@@ -1202,7 +1211,7 @@ void seeded_shape(dinara_batch *b) {
     for (size_t s = 0; s < b->slot_pair.size(); ++s) {
         const int n = b->rows[s], m = b->cols[s];
         const int start = b->start_bin[edit_length_class(std::max(n, m))];
-        const int widest = std::max(start, edit_bin_for(std::min(n + m + 3, edit_width(EDIT_BINS - 1)), 0));
+        const int widest = std::max(start, edit_widest(n + m));
         for (int k = start; k <= widest; ++k) ++first[k];
         for (int k = start + 1; k <= widest; ++k) ++retry[k];
         for (int k = 0; k <= widest; ++k) ++trace[k];
@@ -1451,9 +1460,11 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 9: return b->walked_first;
         default: break;
     }
-    if (what >= 600 && what < 1000 && what % 100 < EDIT_BINS) {
+    if (what >= 600 && what < 1200 && what % 100 < EDIT_BINS) {
         const int bin = (int)(what % 100);
         switch (what / 100) {
+            case 10: return b->sweep_work[bin];
+            case 11: return b->trace_work[bin];
             case 6: return b->edit_ms[bin];
             case 7: return b->edit_sweeps[bin];
             case 8: return b->trace_ms[bin];

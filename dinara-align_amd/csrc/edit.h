@@ -21,11 +21,19 @@
 
 namespace dinara {
 
-// Bins: 1, 2, 4, ..., 64 lanes of one 64-bit word each (64 to 4096 diagonals), then 64 lanes of 2 and 4 words.
+// Bins: 1, 2, 4, ..., 64 lanes of one 64-bit word each (64 to 4096 diagonals), then 64 lanes of 2 and 4 words. Every
+// step is a doubling: a lane keeps one word wherever it can, since the words of a lane are swept one after another
+// and a long pair's column-after-column chain is what the run waits on.
 constexpr int EDIT_BINS = 9;
 __host__ __device__ constexpr int edit_lanes(int bin) { return bin < 7 ? 1 << bin : 64; }
 __host__ __device__ constexpr int edit_words(int bin) { return bin < 7 ? 1 : (bin == 7 ? 2 : 4); }
 __host__ __device__ constexpr int edit_width(int bin) { return 64 * edit_lanes(bin) * edit_words(bin); }
+// The bin twice as wide as `bin` (plain band doubling's next band), -1 past the widest.
+__host__ __device__ inline int edit_doubled(int bin) {
+    for (int next = bin + 1; next < EDIT_BINS; ++next)
+        if (edit_width(next) == 2 * edit_width(bin)) return next;
+    return -1;
+}
 
 struct EditArgs {
     SweepArgs s;            // tape, results, certificate policy (unit costs), CIGAR output
@@ -55,6 +63,7 @@ struct EditArgs {
     unsigned long long arena_capacity;
     uint32_t *stats;        // sweeps begun by bin, then trace sweeps by bin, then pairs their first band left unproven,
                             // then pairs proven with their decisions kept
+    unsigned long long *work;  // word-columns swept (a pair's columns times its band's words), by bin, then traced
     int one_pair;           // 1: each pair group sweeps one pair and stops (no refill), for comparison
     int keep_first;         // 1 (alignments): every sweep keeps its decisions, and a pair it proves walks them
 };
@@ -77,6 +86,17 @@ __host__ __device__ inline int edit_bin_for(int need, int from) {
     for (int bin = from; bin < EDIT_BINS; ++bin)
         if (need <= edit_width(bin)) return bin;
     return -1;
+}
+
+// The widest bin a pair of `letters` letters (both sequences) can reach under either retry policy: a band placed
+// inside the matrix's diagonals and letters + 3 wide proves any pair, and band doubling reaches the first power
+// of two at least that wide.
+__host__ __device__ inline int edit_widest(int letters) {
+    const int need = letters + 3 < edit_width(EDIT_BINS - 1) ? letters + 3 : edit_width(EDIT_BINS - 1);
+    int power = 64;
+    while (power < need) power *= 2;
+    const int a = edit_bin_for(need, 0), b = edit_bin_for(power, 0);
+    return a > b ? a : b;
 }
 
 // Where a band of `width` diagonals holding [want_low, want_high] starts: centred, kept inside the matrix's

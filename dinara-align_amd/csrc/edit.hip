@@ -92,10 +92,13 @@ struct Cursor {
     }
 };
 
-// Where the four columns 4q..4q+3 of word w of a pair's `words` words lie in its kept decisions: eight columns of a
-// word take 128 contiguous bytes (two 64-bit planes a column), the words of an eight-column block side by side.
-__device__ __forceinline__ size_t trace_chunk(int q, int words, int w) {
-    return ((size_t)(q >> 1) * words + w) * 128 + (size_t)(q & 1) * 64;
+
+// Where the HOLD columns HOLD q .. HOLD q + HOLD - 1 of word w of a pair's `words` words lie in its kept decisions:
+// eight columns of a word take 128 contiguous bytes (two 64-bit planes a column), the words of an eight-column block
+// side by side; HOLD is 8 (a whole block) or 4 (half of one).
+template <int HOLD>
+__device__ __forceinline__ size_t held_chunk_at(int q, int words, int w) {
+    return HOLD == 8 ? ((size_t)q * words + w) * 128 : ((size_t)(q >> 1) * words + w) * 128 + (size_t)(q & 1) * 64;
 }
 
 // ------------------------------------------------------------------ settling and filing (first lanes only)
@@ -139,8 +142,7 @@ __device__ void refile(const EditArgs &a, int slot, int n, int m, int lo, int hi
         want_low = lo;
         want_high = hi;
     }
-    const int next = a.s.doubling ? (bin + 1 < EDIT_BINS ? bin + 1 : -1)
-                                  : edit_bin_for(want_high - want_low + 1, bin + 1);
+    const int next = a.s.doubling ? edit_doubled(bin) : edit_bin_for(want_high - want_low + 1, bin + 1);
     if (next < 0) {
         a.s.status[slot] = 2;
         return;
@@ -173,6 +175,14 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     int32_t *counts = TRACE ? a.trace_counts : a.counts;
     const int count = counts[bin];
     int32_t *head = counts + EDIT_BINS + bin;
+    // Wide bins hold the longest pairs, whose column after column is the run's longest chain: their waves issue
+    // first when they share a SIMD with narrower bins' waves.
+    if (W >= 2048)
+        __builtin_amdgcn_s_setprio(3);
+    else if (W >= 512)
+        __builtin_amdgcn_s_setprio(2);
+    else if (W >= 128)
+        __builtin_amdgcn_s_setprio(1);
 
     int slot = -1;
     bool drained = false;
@@ -185,9 +195,12 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     bool recording = false;    // this group keeps the current pair's decisions
     long long trace_at = 0;    // and where, from the arena's start
     constexpr int WL = L * K;  // the pair's words
-    uint64_t held0[4][K], held1[4][K];  // the last four columns' decision planes, oldest first
+    // The last HOLD columns' decision planes, oldest first: eight (a whole 128-byte line a word) when a lane holds one
+    // word, four (half a line) when it holds more.
+    constexpr int HOLD = K == 1 ? 8 : 4;
+    uint64_t held0[HOLD][K], held1[HOLD][K];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < HOLD; ++q)
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) held0[q][kk] = held1[q][kk] = 0;
 #pragma unroll
@@ -253,7 +266,10 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                         recording = trace_at >= 0;
                     }
                     trace = recording ? a.arena + trace_at : a.arena;
-                    if (first_lane) atomicAdd(&a.stats[(TRACE ? EDIT_BINS : 0) + bin], 1u);
+                    if (first_lane) {
+                        atomicAdd(&a.stats[(TRACE ? EDIT_BINS : 0) + bin], 1u);
+                        atomicAdd(&a.work[(TRACE ? EDIT_BINS : 0) + bin], (unsigned long long)m * (L * K));
+                    }
                 } else {
                     drained = true;
                 }
@@ -280,24 +296,23 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             for (int d = L / 2; d >= 1; d /= 2) part += __shfl_xor(part, d);
             const int found = hi + m + top + part;  // `top` is the first lane's
             if constexpr (RECORD) {
-                // The last, partly filled four columns: shifted down to the chunk's start, then written whole.
-                if (finishing && recording && (m & 3)) {
-                    for (int pad = m & 3; pad < 4; ++pad) {
+                // The last, partly filled columns: shifted down to the chunk's start, then written whole.
+                if (finishing && recording && (m % HOLD)) {
+                    for (int pad = m % HOLD; pad < HOLD; ++pad) {
 #pragma unroll
                         for (int kk = 0; kk < K; ++kk) {
-                            held0[0][kk] = held0[1][kk];
-                            held0[1][kk] = held0[2][kk];
-                            held0[2][kk] = held0[3][kk];
-                            held1[0][kk] = held1[1][kk];
-                            held1[1][kk] = held1[2][kk];
-                            held1[2][kk] = held1[3][kk];
+#pragma unroll
+                            for (int q = 0; q + 1 < HOLD; ++q) {
+                                held0[q][kk] = held0[q + 1][kk];
+                                held1[q][kk] = held1[q + 1][kk];
+                            }
                         }
                     }
 #pragma unroll
                     for (int kk = 0; kk < K; ++kk) {
-                        ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + trace_chunk(m >> 2, WL, k * K + kk));
+                        ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + held_chunk_at<HOLD>(m / HOLD, WL, k * K + kk));
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
+                        for (int q = 0; q < HOLD; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
                     }
                 }
             }
@@ -414,18 +429,17 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             if constexpr (RECORD) {
                 // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
                 const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
-                held0[0][kk] = held0[1][kk];
-                held0[1][kk] = held0[2][kk];
-                held0[2][kk] = held0[3][kk];
-                held0[3][kk] = ~unequal_step & (npv | ~ph[kk]);
-                held1[0][kk] = held1[1][kk];
-                held1[1][kk] = held1[2][kk];
-                held1[2][kk] = held1[3][kk];
-                held1[3][kk] = ~unequal_step & ~npv;
-                if (active && recording && (j & 3) == 0) {
-                    ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + trace_chunk((j >> 2) - 1, WL, k * K + kk));
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
+                for (int q = 0; q + 1 < HOLD; ++q) {
+                    held0[q][kk] = held0[q + 1][kk];
+                    held1[q][kk] = held1[q + 1][kk];
+                }
+                held0[HOLD - 1][kk] = ~unequal_step & (npv | ~ph[kk]);
+                held1[HOLD - 1][kk] = ~unequal_step & ~npv;
+                if (active && recording && (j % HOLD) == 0) {
+                    ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + held_chunk_at<HOLD>(j / HOLD - 1, WL, k * K + kk));
+#pragma unroll
+                    for (int q = 0; q < HOLD; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
                 }
             }
             pv[kk] = npv;
@@ -438,12 +452,31 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 
 // ------------------------------------------------------------------ the walk
 
-// One thread a queued pair: from (n, m) back to the origin over the 2-bit decisions, then the CIGAR text.
+// One thread a queued pair: from (n, m) back to the origin over the 2-bit decisions, then the CIGAR text. Each
+// thread keeps the 128 bytes it reads from (eight columns of one word) in its own slot of LDS, so a step reads its
+// cell by address, and makes its move without branching: the move's code from one table, its row and column steps
+// from the choice.
 __global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
+    __shared__ ulonglong2 held[256][8];
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int count = *a.walk_count;
     if ((t & ~63) >= count) return;  // whole waves only: publishing takes every lane of a wave
     const int slot = t < count ? a.walk_list[t] : -1;
+    // A wave walking a long pair issues first: its steps, one after another, are the walk's longest chain.
+    {
+        int letters = slot >= 0 ? a.s.tape.rows[slot] + a.s.tape.cols[slot] : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int other = __shfl_xor(letters, d);
+            letters = other > letters ? other : letters;
+        }
+        if (letters > 16000)
+            __builtin_amdgcn_s_setprio(3);
+        else if (letters > 6000)
+            __builtin_amdgcn_s_setprio(2);
+        else if (letters > 2000)
+            __builtin_amdgcn_s_setprio(1);
+    }
     int64_t moves = -1;
     if (slot >= 0) {
         const int n = a.s.tape.rows[slot], m = a.s.tape.cols[slot];
@@ -451,14 +484,22 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
         const int hi = a.s.walk_low[slot] + edit_width(bin) - 1;
         const uint8_t *base = a.arena + (n > 0 && m > 0 ? a.trace_offset[slot] : 0);
         OpTape ops = op_tape(a.s, slot);
-        const uint32_t unequal = a.s.eqx ? 1u : 0u;
+        // The move each choice makes, 2 bits each: choice 0 (two unequal letters) 'X' (or '=' shared with 'M'
+        // without eqx), 1 'D', 2 'I', 3 two equal letters.
+        const uint32_t move_of = (a.s.eqx ? 1u : 0u) | (2u << 2) | (3u << 4);
         const int width = edit_width(bin);
         const int words = edit_lanes(bin) * edit_words(bin);
-        int r = n, c = m;
-        // The 128 bytes held: eight columns of one word (as eight-column block times words + word), plane 0 of
-        // column k in h[k], plane 1 in g[k].
+        ulonglong2 *mine = held[threadIdx.x];
         int held_chunk = -1;
-        uint64_t h[8] = {}, g[8] = {};
+        // The chunk a diagonal run reaches next (the same word, eight columns back), loaded into registers as soon
+        // as the walk enters the one before it, and written to LDS only when the walk gets there.
+        int next_chunk = -1;
+        ulonglong2 n0{}, n1{}, n2{}, n3{}, n4{}, n5{}, n6{}, n7{};
+        auto fetch = [&](int chunk) {
+            const ulonglong2 *at = reinterpret_cast<const ulonglong2 *>(base + (size_t)chunk * 128);
+            n0 = at[0], n1 = at[1], n2 = at[2], n3 = at[3], n4 = at[4], n5 = at[5], n6 = at[6], n7 = at[7];
+        };
+        int r = n, c = m;
         while (r > 0 && c > 0) {
             const int i = hi - c + r;
             if (i < 0 || i >= width) {  // a proven band holds every step; this pair's did not
@@ -467,34 +508,19 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
             }
             const int want = ((c - 1) >> 3) * words + (i >> 6);
             if (want != held_chunk) {
-                const uint64_t *at = reinterpret_cast<const uint64_t *>(base + (size_t)want * 128);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    h[k] = at[2 * k];
-                    g[k] = at[2 * k + 1];
-                }
+                if (want != next_chunk) fetch(want);
+                mine[0] = n0, mine[1] = n1, mine[2] = n2, mine[3] = n3;
+                mine[4] = n4, mine[5] = n5, mine[6] = n6, mine[7] = n7;
                 held_chunk = want;
+                next_chunk = want - words;
+                if (next_chunk >= 0) fetch(next_chunk);
             }
-            const int pos = (c - 1) & 7;
-            uint64_t plane0 = h[0], plane1 = g[0];
-#pragma unroll
-            for (int k = 1; k < 8; ++k) {
-                plane0 = pos == k ? h[k] : plane0;
-                plane1 = pos == k ? g[k] : plane1;
-            }
+            const ulonglong2 cell = mine[(c - 1) & 7];
             const uint32_t bit = i & 63;
-            const uint32_t code = (uint32_t)((plane0 >> bit) & 1) | ((uint32_t)((plane1 >> bit) & 1) << 1);
-            if (code == 1) {
-                ops.put(2u);
-                --r;
-            } else if (code == 2) {
-                ops.put(3u);
-                --c;
-            } else {
-                ops.put(code == 0 ? unequal : 0u);
-                --r;
-                --c;
-            }
+            const uint32_t choice = (uint32_t)((cell.x >> bit) & 1) | ((uint32_t)((cell.y >> bit) & 1) << 1);
+            ops.put((move_of >> (2 * choice)) & 3u);
+            r -= choice != 2u;
+            c -= choice != 1u;
         }
         for (; r > 0; --r) ops.put(2u);
         for (; c > 0; --c) ops.put(3u);
@@ -525,6 +551,7 @@ __global__ void edit_reset(EditArgs a, int slots) {
         a.trace_counts[EDIT_BINS + t] = 0;
     }
     if (t <= 2 * EDIT_BINS + 1) a.stats[t] = 0;
+    if (t < 2 * EDIT_BINS) a.work[t] = 0;
     if (t == 0) {
         *a.walk_count = 0;
         *a.deferred_count = 0;

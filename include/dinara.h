@@ -205,8 +205,9 @@ void dinara_batch_free(dinara_batch *batch);
  * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
  * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep); for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
- * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began. For roofline
- * reporting. No counterpart in the reference. */
+ * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began, 1000+b and
+ * 1100+b the word-columns (a pair's columns times its band's 64-bit words) its sweeps and traces covered. For
+ * roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
 /*

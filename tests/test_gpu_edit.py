@@ -21,6 +21,8 @@ from oracle import oracle as o
 pytestmark = pytest.mark.gpu
 
 EDIT = o.Costs.affine(1, 0, 1)
+# Band bins: 64, 128, 256, ..., 4096, 8192, 16384 diagonals.
+BINS = 9
 
 
 def mutated(text, rate, rng, alphabet="ACGT"):
@@ -119,7 +121,7 @@ def test_bins_of_several_lanes_against_the_oracle(right):
     b = Batch(refs, qrys, 1, ties_right=right)
     try:
         got = b.run()
-        swept = [b.stat(700 + k) for k in range(9)]
+        swept = [b.stat(700 + k) for k in range(BINS)]
     finally:
         b.free()
     assert got == oracle_alignments(refs, qrys, right=right)
@@ -135,7 +137,7 @@ def test_words_of_two_and_four_a_lane():
     b = Batch(refs, qrys, 0)
     try:
         got = b.run()
-        swept = [b.stat(700 + k) for k in range(9)]
+        swept = [b.stat(700 + k) for k in range(BINS)]
     finally:
         b.free()
     assert got == [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
@@ -146,7 +148,7 @@ def test_words_of_two_and_four_a_lane():
     b = Batch([r, r], qs, 1)
     try:
         got = b.run()
-        traced = [b.stat(900 + k) for k in range(9)]
+        traced = [b.stat(900 + k) for k in range(BINS)]
     finally:
         b.free()
     assert got == oracle_alignments([r, r], qs)
@@ -287,7 +289,7 @@ def test_autotune_keeps_every_result():
     try:
         chosen = (ctypes.c_int64 * 7)()
         assert h.dinara_batch_autotune(b.ptr, chosen) == 0
-        assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < 9 for i in range(1, 6))
+        assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < BINS for i in range(1, 6))
         got = b.run()
     finally:
         b.free()
