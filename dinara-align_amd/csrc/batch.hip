@@ -207,8 +207,9 @@ struct dinara_batch {
     int64_t trace_reach[EDIT_BINS] = {};  // pairs a bin's tracing sweep may meet
     DeviceBuffer features_d, predictions_d;  // learned seeding on the device
     int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
-    DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
-        trace_counts, trace_offset, walk_list, walk_count, deferred, deferred_count, deferred_held, edit_arena,
+    DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_later_lists,
+        edit_later_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
+        trace_counts, trace_offset, walk_ready, deferred, deferred_count, deferred_held, edit_arena,
         arena_used, edit_stats, edit_work;
     size_t arena_capacity = 0;
     hipGraphExec_t edit_graph = nullptr;
@@ -558,8 +559,9 @@ int64_t unit_lists(dinara_batch *b) {
     }
     // The captured graph reads these buffers where they were: it is kept unless one moves.
     DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
-                            &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
-                            &b->walk_list, &b->walk_count, &b->deferred, &b->deferred_count, &b->deferred_held,
+                            &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts,
+                            &b->edit_later_lists, &b->edit_later_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
+                            &b->walk_ready, &b->deferred, &b->deferred_count, &b->deferred_held,
                             &b->arena_used, &b->edit_stats, &b->edit_work};
     void *before[sizeof(held) / sizeof(held[0])];
     for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i) before[i] = held[i]->ptr;
@@ -580,8 +582,9 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->edit_first_low_d.upload(ctx, b->edit_first_low.data(), cap * 4) || !b->edit_band_low.reserve(cap * 4) ||
         !b->edit_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->edit_retry_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_retry_counts.reserve(2 * EDIT_BINS * 4) ||
+        !b->edit_later_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_later_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
-        !b->trace_offset.reserve(cap * 8) || !b->walk_list.reserve(cap * 4) || !b->walk_count.reserve(4) ||
+        !b->trace_offset.reserve(cap * 8) || !b->walk_ready.reserve(cap * 4) ||
         !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
         !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4) ||
         !b->edit_work.reserve(2 * EDIT_BINS * 8))
@@ -603,6 +606,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.counts = static_cast<int32_t *>(b->edit_counts.ptr);
     e.file_lists = static_cast<int32_t *>(b->edit_retry_lists.ptr);
     e.file_counts = static_cast<int32_t *>(b->edit_retry_counts.ptr);
+    e.spare_counts = static_cast<int32_t *>(b->edit_later_counts.ptr);
     e.first_lists = static_cast<int32_t *>(b->edit_first_lists.ptr);
     e.first_counts = static_cast<const int32_t *>(b->edit_first_counts.ptr);
     e.first_counts_out = static_cast<int32_t *>(b->edit_first_counts.ptr);
@@ -610,8 +614,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.trace_lists = static_cast<int32_t *>(b->trace_lists.ptr);
     e.trace_counts = static_cast<int32_t *>(b->trace_counts.ptr);
     e.trace_offset = static_cast<int64_t *>(b->trace_offset.ptr);
-    e.walk_list = static_cast<int32_t *>(b->walk_list.ptr);
-    e.walk_count = static_cast<int32_t *>(b->walk_count.ptr);
+    e.walk_ready = static_cast<int32_t *>(b->walk_ready.ptr);
     e.deferred = static_cast<int32_t *>(b->deferred.ptr);
     e.deferred_count = static_cast<int32_t *>(b->deferred_count.ptr);
     e.arena = static_cast<uint8_t *>(b->edit_arena.ptr);
@@ -652,7 +655,8 @@ size_t arena_bytes(const dinara_batch *b) {
 
 // Timing events of a run launched kernel by kernel: a pair for each bin's first sweep, retry sweep and tracing
 // sweep, and for the walk.
-constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, EV_WALK = 6 * EDIT_BINS;
+constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, EV_WALK = 6 * EDIT_BINS,
+              EV_LATER = 6 * EDIT_BINS + 2;
 
 // Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
 // stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
@@ -691,15 +695,26 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace)
                  [&](int bin, hipStream_t st) {
                      launch_edit_sweep(a, bin, false, edit_grid(b, bin, b->first_count[bin]), st);
                  });
+    // The first retries side by side, filing into the later set; the later retries one bin after another.
     EditArgs retry = a;
     retry.lists = a.file_lists;
     retry.counts = a.file_counts;
+    retry.file_lists = static_cast<int32_t *>(b->edit_later_lists.ptr);
+    retry.file_counts = static_cast<int32_t *>(b->edit_later_counts.ptr);
+    side_by_side(ctx, 3 * EDIT_BINS + 2, timed, EV_RETRY, b->side_by_side_bins,
+                 [&](int bin) { return b->retry_reach[bin] > 0; },
+                 [&](int bin, hipStream_t st) {
+                     launch_edit_sweep(retry, bin, false, edit_grid(b, bin, b->retry_reach[bin]), st);
+                 });
+    EditArgs later = retry;
+    later.lists = retry.file_lists;
+    later.counts = retry.file_counts;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
         const int grid = edit_grid(b, bin, b->retry_reach[bin]);
         if (!grid) continue;
-        if (timed) (void)hipEventRecord(ctx->event(EV_RETRY + 2 * bin), ctx->stream);
-        launch_edit_sweep(retry, bin, false, grid, ctx->stream);
-        if (timed) (void)hipEventRecord(ctx->event(EV_RETRY + 2 * bin + 1), ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin), ctx->stream);
+        launch_edit_sweep(later, bin, false, grid, ctx->stream);
+        if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin + 1), ctx->stream);
     }
     if (trace) {
         side_by_side(ctx, EDIT_BINS + 1, timed, EV_TRACE, b->side_by_side_bins,
@@ -805,6 +820,9 @@ int64_t run_unit(dinara_batch *b, bool trace) {
                 b->edit_ms[bin] += ms;
             if (b->retry_reach[bin] > 0 &&
                 hipEventElapsedTime(&ms, ctx->event(EV_RETRY + 2 * bin), ctx->event(EV_RETRY + 2 * bin + 1)) == hipSuccess)
+                b->edit_ms[bin] += ms;
+            if (b->retry_reach[bin] > 0 &&
+                hipEventElapsedTime(&ms, ctx->event(EV_LATER + 2 * bin), ctx->event(EV_LATER + 2 * bin + 1)) == hipSuccess)
                 b->edit_ms[bin] += ms;
             if (trace && b->trace_reach[bin] > 0 &&
                 hipEventElapsedTime(&ms, ctx->event(EV_TRACE + 2 * bin), ctx->event(EV_TRACE + 2 * bin + 1)) == hipSuccess)

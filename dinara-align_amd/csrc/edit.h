@@ -13,8 +13,9 @@
 // group takes its next pair from its bin's queue the moment it finishes the last one (a persistent launch): pairs of
 // very different lengths share waves without waiting for each other. Every bin's first queue is swept at once, the
 // bins side by side on streams of their own (a long pair's tail in one bin overlaps the others' work); a pair its
-// band does not prove is filed into a retry queue of a wider bin, and the retry queues are swept after, narrowest
-// bin first. The launches have a fixed shape, so one hipGraph replays them.
+// band does not prove is filed into a retry queue of a wider bin. The retry queues are swept side by side too,
+// filing into a second retry set, which is swept last, narrowest bin first. The launches have a fixed shape, so one
+// hipGraph replays them.
 #pragma once
 
 #include "common.h"
@@ -45,6 +46,7 @@ struct EditArgs {
     int32_t *counts;        // [0, EDIT_BINS) queue lengths, [EDIT_BINS, 2 EDIT_BINS) queue heads
     int32_t *file_lists;    // the queues a launch files unproven pairs into (laid out as `lists`)
     int32_t *file_counts;
+    int32_t *spare_counts;  // a further queue set's counts, emptied with `file_counts` at the start of a run
     int32_t *first_lists;   // the first sweep's queues, copied into `lists` at the start of a run
     const int32_t *first_counts;
     int32_t *first_counts_out;  // the same counts, written by the seeding kernel
@@ -54,8 +56,7 @@ struct EditArgs {
     int32_t *trace_lists;   // EDIT_BINS queues of `capacity` slots
     int32_t *trace_counts;  // lengths, then heads
     int64_t *trace_offset;  // per slot: byte offset of its decisions in the arena
-    int32_t *walk_list;
-    int32_t *walk_count;
+    int32_t *walk_ready;    // per slot: 1 when its decisions are kept and it waits for the walk
     int32_t *deferred;      // settled pairs whose decisions found no room in the arena this pass
     int32_t *deferred_count;
     uint8_t *arena;

@@ -111,7 +111,7 @@ __device__ void settle(const EditArgs &a, int slot, int n, int m, int64_t found)
     if (n == 0 || m == 0) {                  // all gaps: nothing to trace
         s.walk_low[slot] = 0;
         s.walk_class[slot] = 0;
-        a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+        a.walk_ready[slot] = 1;
         return;
     }
     // The narrowest band the strict certificate proves at the found cost: it holds every optimal path.
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                 if (TRACE) {
                     // The traced band was proven in the first pass: its cost must agree.
                     if ((int64_t)found * a.scale != (int64_t)a.s.cost[slot]) a.s.status[slot] = 3;
-                    a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+                    a.walk_ready[slot] = 1;
                 } else {
                     int want_low = 0, want_high = 0;
                     if (!certify(a.s, n, m, found, lo, hi, &want_low, &want_high)) {
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                         a.s.walk_low[slot] = lo;
                         a.s.walk_class[slot] = bin;
                         a.trace_offset[slot] = trace_at;
-                        a.walk_list[atomicAdd(a.walk_count, 1)] = slot;
+                        a.walk_ready[slot] = 1;
                         atomicAdd(&a.stats[2 * EDIT_BINS + 1], 1u);
                     } else {
                         settle(a, slot, n, m, found);
@@ -452,16 +452,17 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 
 // ------------------------------------------------------------------ the walk
 
-// One thread a queued pair: from (n, m) back to the origin over the 2-bit decisions, then the CIGAR text. Each
-// thread keeps the 128 bytes it reads from (eight columns of one word) in its own slot of LDS, so a step reads its
-// cell by address, and makes its move without branching: the move's code from one table, its row and column steps
-// from the choice.
-__global__ __launch_bounds__(256) void edit_walk(EditArgs a) {
+// One thread a slot, walking it if it is ready: from (n, m) back to the origin over the 2-bit decisions, then the
+// CIGAR text. Slots are in length order, longest first, so a wave's pairs walk about as far as each other and the
+// long walks share their waves and SIMDs with each other, not with short ones. Each thread keeps the 128 bytes it
+// reads from (eight columns of one word) in its own slot of LDS, so a step reads its cell by address, and makes its
+// move without branching: the move's code from one table, its row and column steps from the choice.
+__global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
     __shared__ ulonglong2 held[256][8];
     const int t = blockIdx.x * 256 + threadIdx.x;
-    const int count = *a.walk_count;
-    if ((t & ~63) >= count) return;  // whole waves only: publishing takes every lane of a wave
-    const int slot = t < count ? a.walk_list[t] : -1;
+    if ((t & ~63) >= slots) return;  // whole waves only: publishing takes every lane of a wave
+    const int slot = t < slots && a.walk_ready[t] ? t : -1;
+    if (slot >= 0) a.walk_ready[slot] = 0;
     // A wave walking a long pair issues first: its steps, one after another, are the walk's longest chain.
     {
         int letters = slot >= 0 ? a.s.tape.rows[slot] + a.s.tape.cols[slot] : 0;
@@ -538,6 +539,7 @@ __global__ void edit_reset(EditArgs a, int slots) {
     if (t < slots) {
         a.s.status[t] = -1;
         a.band_low[t] = a.first_band_low[t];
+        a.walk_ready[t] = 0;
     }
     for (int bin = 0; bin < EDIT_BINS; ++bin)
         if (t < a.first_counts[bin])
@@ -547,13 +549,14 @@ __global__ void edit_reset(EditArgs a, int slots) {
         a.counts[EDIT_BINS + t] = 0;
         a.file_counts[t] = 0;
         a.file_counts[EDIT_BINS + t] = 0;
+        a.spare_counts[t] = 0;
+        a.spare_counts[EDIT_BINS + t] = 0;
         a.trace_counts[t] = 0;
         a.trace_counts[EDIT_BINS + t] = 0;
     }
     if (t <= 2 * EDIT_BINS + 1) a.stats[t] = 0;
     if (t < 2 * EDIT_BINS) a.work[t] = 0;
     if (t == 0) {
-        *a.walk_count = 0;
         *a.deferred_count = 0;
         *a.arena_used = 0;
         if (a.s.cigar_used) *a.s.cigar_used = 0;
@@ -565,7 +568,6 @@ __global__ void edit_clear(EditArgs a) {
     const int t = threadIdx.x;
     if (t < 2 * EDIT_BINS) a.trace_counts[t] = 0;
     if (t == 0) {
-        *a.walk_count = 0;
         *a.deferred_count = 0;
         *a.arena_used = 0;
     }
@@ -683,7 +685,7 @@ void launch_edit_seed(const EditArgs &a, const float *prediction, double margin,
 }
 
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {
-    hipLaunchKernelGGL(edit_walk, dim3((slots + 255) / 256), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(edit_walk, dim3((slots + 255) / 256), dim3(256), 0, stream, a, slots);
 }
 
 void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream) {
