@@ -654,8 +654,9 @@ def run_uneven(args) -> int:
             raise RuntimeError("training failed")
         train_s = time.time() - t1
         del x_train
-        margin = args.margin
-        seed = lambda: handle.dinara_batch_seed_learned(batch, net, ctypes.c_double(margin))
+        margin, slope = args.margin, args.slope
+        seed = lambda: handle.dinara_batch_seed_learned_by_length(batch, net, ctypes.c_double(margin),
+                                                                  ctypes.c_double(slope))
         # The seeds choose each pair's first band: no start bin holds it wider.
         handle.dinara_batch_set_tuning(batch, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
         seed()
@@ -668,6 +669,12 @@ def run_uneven(args) -> int:
         learned_chosen = (ctypes.c_int64 * 7)()
         if handle.dinara_batch_autotune(batch, learned_chosen) != 0:
             raise RuntimeError("autotune failed")
+        by_slope = {}
+        if args.slopes:
+            for slope in [float(v) for v in args.slopes.split(",")]:
+                by_slope[slope] = timed_steps(handle, batch, 2, 1, before=seed)[0]
+                print(f"[bench] slope {slope}: {by_slope[slope]:.1f} ms", file=sys.stderr, flush=True)
+            slope = min(by_slope, key=by_slope.get)
         learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
         results["learned"] = edit_results(handle, batch, A, pairs)
         learned_proven = handle.dinara_batch_stat(batch, 4)
@@ -675,7 +682,8 @@ def run_uneven(args) -> int:
         learned_prof = bin_profile(handle, batch)
         learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
                         "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
-                        "margin": margin, "first_band_proved": round(learned_proven / pairs, 4),
+                        "margin": margin, "slope": slope, "by_slope_ms": {str(k): round(v, 3) for k, v in by_slope.items()},
+                        "first_band_proved": round(learned_proven / pairs, 4),
                         "walked_from_first_sweep": round(walked_first / pairs, 4),
                         "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
                         "tuning": {"grid_per_cu": int(learned_chosen[0]), "side_by_side": int(learned_chosen[6])}}
@@ -862,7 +870,10 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=None, help="learned: training epochs (default 3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="uneven: pairs the CPU baseline times")
-    ap.add_argument("--margin", type=float, default=0.0, help="uneven: learned seeds e^(prediction + margin) - 1")
+    ap.add_argument("--margin", type=float, default=0.0,
+                    help="uneven: learned seeds e^(prediction + margin + slope ln(max(longer, 1000) / 1000)) - 1")
+    ap.add_argument("--slope", type=float, default=0.0, help="uneven: the seeds' margin per e-fold of length")
+    ap.add_argument("--slopes", default="", help="uneven: comma-separated slopes to time, keeping the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "

@@ -173,12 +173,8 @@ __device__ __forceinline__ int64_t walk_tiles(const SweepArgs &a, int slot, int 
 __device__ __forceinline__ void publish_cigars(const SweepArgs &a, int slot, int64_t moves) {
     const uint64_t *words = reinterpret_cast<const uint64_t *>(
         a.cigar_scratch + (moves >= 0 ? a.cigar_scratch_base[slot] : 0));
-    const int64_t length = moves >= 0 ? cigar_text(a.eqx, a.forward_cigar, words, moves, nullptr) : 0;
-    const unsigned long long at = wave_claim(a.cigar_used, (unsigned long long)length);
-    if (moves < 0) return;
-    cigar_text(a.eqx, a.forward_cigar, words, moves, a.cigars + at);
-    a.cigar_offset[slot] = (int64_t)at;
-    a.cigar_length[slot] = (int32_t)length;
+    publish_wave(a.eqx, a.forward_cigar, words, moves, a.cigars, a.cigar_used, a.cigar_offset + slot,
+                 a.cigar_length + slot);
 }
 
 }  // namespace dinara

@@ -209,7 +209,7 @@ struct dinara_batch {
     int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
     DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_later_lists,
         edit_later_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
-        trace_counts, trace_offset, walk_ready, deferred, deferred_count, deferred_held, edit_arena,
+        trace_counts, trace_offset, walk_ready, walk_moves, deferred, deferred_count, deferred_held, edit_arena,
         arena_used, edit_stats, edit_work;
     size_t arena_capacity = 0;
     hipGraphExec_t edit_graph = nullptr;
@@ -561,7 +561,7 @@ int64_t unit_lists(dinara_batch *b) {
     DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
                             &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts,
                             &b->edit_later_lists, &b->edit_later_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
-                            &b->walk_ready, &b->deferred, &b->deferred_count, &b->deferred_held,
+                            &b->walk_ready, &b->walk_moves, &b->deferred, &b->deferred_count, &b->deferred_held,
                             &b->arena_used, &b->edit_stats, &b->edit_work};
     void *before[sizeof(held) / sizeof(held[0])];
     for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i) before[i] = held[i]->ptr;
@@ -585,6 +585,7 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->edit_later_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_later_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_offset.reserve(cap * 8) || !b->walk_ready.reserve(cap * 4) ||
+        !b->walk_moves.reserve(cap * 4) ||
         !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
         !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4) ||
         !b->edit_work.reserve(2 * EDIT_BINS * 8))
@@ -615,6 +616,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.trace_counts = static_cast<int32_t *>(b->trace_counts.ptr);
     e.trace_offset = static_cast<int64_t *>(b->trace_offset.ptr);
     e.walk_ready = static_cast<int32_t *>(b->walk_ready.ptr);
+    e.walk_moves = static_cast<int32_t *>(b->walk_moves.ptr);
     e.deferred = static_cast<int32_t *>(b->deferred.ptr);
     e.deferred_count = static_cast<int32_t *>(b->deferred_count.ptr);
     e.arena = static_cast<uint8_t *>(b->edit_arena.ptr);
@@ -1248,7 +1250,7 @@ void seeded_shape(dinara_batch *b) {
 
 // Learned seeds without leaving the device (unit-cost batches): features, the network's predictions and each
 // pair's first band and queue, all on the batch's stream.
-int64_t seed_unit_learned(dinara_batch *b, dinara_mlp *net, double margin) {
+int64_t seed_unit_learned(dinara_batch *b, dinara_mlp *net, double margin, double slope) {
     const int64_t slots = (int64_t)b->slot_pair.size();
     if (slots == 0) return 0;
     DeviceContext *ctx = b->ctx;
@@ -1264,8 +1266,8 @@ int64_t seed_unit_learned(dinara_batch *b, dinara_mlp *net, double margin) {
         return code;
     EditStart start{};
     for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) start.bin[c] = b->start_bin[c];
-    launch_edit_seed(edit_args(b), static_cast<const float *>(b->predictions_d.ptr), margin, start, (int)slots,
-                     ctx->stream);
+    launch_edit_seed(edit_args(b), static_cast<const float *>(b->predictions_d.ptr), margin, slope, start,
+                     (int)slots, ctx->stream);
     seeded_shape(b);
     return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
 }
@@ -1373,8 +1375,12 @@ int64_t dinara_batch_seed(dinara_batch *b, const int64_t *predicted) {
 }
 
 int64_t dinara_batch_seed_learned(dinara_batch *b, dinara_mlp *net, double margin) {
+    return dinara_batch_seed_learned_by_length(b, net, margin, 0.0);
+}
+
+int64_t dinara_batch_seed_learned_by_length(dinara_batch *b, dinara_mlp *net, double margin, double slope) {
     if (!b || !net) return DINARA_INVALID_MODE;
-    if (b->unit) return seed_unit_learned(b, net, margin);
+    if (b->unit) return seed_unit_learned(b, net, margin, slope);
     const int64_t slots = (int64_t)b->slot_pair.size();
     if (slots == 0) return 0;
     std::vector<float> predicted((size_t)slots);
@@ -1392,7 +1398,8 @@ int64_t dinara_batch_seed_learned(dinara_batch *b, dinara_mlp *net, double margi
             return code;
     }
     return seed_slots(b, [&](int64_t s) {
-        const double guess = std::expm1((double)predicted[s] + margin);
+        const double longer = std::max(1000.0, (double)std::max(b->rows[s], b->cols[s]));
+        const double guess = std::expm1((double)predicted[s] + margin + slope * std::log(longer / 1000.0));
         return guess > 0 ? (int64_t)guess : int64_t(0);
     });
 }

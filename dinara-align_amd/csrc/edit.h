@@ -57,6 +57,7 @@ struct EditArgs {
     int32_t *trace_counts;  // lengths, then heads
     int64_t *trace_offset;  // per slot: byte offset of its decisions in the arena
     int32_t *walk_ready;    // per slot: 1 when its decisions are kept and it waits for the walk
+    int32_t *walk_moves;    // per slot: the moves the last walk made (-1: not walked by it), for its CIGAR text
     int32_t *deferred;      // settled pairs whose decisions found no room in the arena this pass
     int32_t *deferred_count;
     uint8_t *arena;
@@ -121,8 +122,8 @@ void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream);
 void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream);
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream);
 // Every slot's first band and first queue from predictions of log(1 + cost) in device memory (learned seeds).
-void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, const EditStart &start, int slots,
-                      hipStream_t stream);
+void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, double slope, const EditStart &start,
+                      int slots, hipStream_t stream);
 // Empties the arena and the trace and walk queues, then queues `count` deferred pairs (`held`, a copy of the
 // deferred list) for a further tracing pass.
 void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream);

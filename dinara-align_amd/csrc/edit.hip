@@ -452,18 +452,46 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 
 // ------------------------------------------------------------------ the walk
 
-// One thread a slot, walking it if it is ready: from (n, m) back to the origin over the 2-bit decisions, then the
-// CIGAR text. Slots are in length order, longest first, so a wave's pairs walk about as far as each other and the
-// long walks share their waves and SIMDs with each other, not with short ones. Each thread keeps the 128 bytes it
-// reads from (eight columns of one word) in its own slot of LDS, so a step reads its cell by address, and makes its
-// move without branching: the move's code from one table, its row and column steps from the choice.
+// The 2-bit choices of one diagonal (bit `bit` of the word) in the eight columns of a chunk, column q's in bits 2q
+// and 2q + 1.
+__device__ __forceinline__ uint32_t diagonal_choices(const ulonglong2 (&chunk)[8], int bit) {
+    const bool high = bit >= 32;
+    const uint32_t shift = (uint32_t)bit & 31u;
+    uint32_t packed = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint32_t x = high ? (uint32_t)(chunk[q].x >> 32) : (uint32_t)chunk[q].x;
+        const uint32_t y = high ? (uint32_t)(chunk[q].y >> 32) : (uint32_t)chunk[q].y;
+        packed |= ((x >> shift) & 1u) << (2 * q);
+        packed |= ((y >> shift) & 1u) << (2 * q + 1);
+    }
+    return packed;
+}
+
+__device__ __forceinline__ void load_chunk(ulonglong2 (&to)[8], const uint8_t *from) {
+    const ulonglong2 *at = reinterpret_cast<const ulonglong2 *>(from);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) to[q] = at[q];
+}
+
+// What a lane with no chunk to walk loads, so that every lane makes the same loads.
+__device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (one address space, global loads)
+
+// One thread a slot, walking it if it is ready: from (n, m) back to the origin over the 2-bit decisions. Slots are
+// in length order, longest first, so a wave's pairs walk about as far as each other.
+//
+// The walk goes by rounds, the wave's lanes together. In a round a lane stays in one chunk (eight columns of one
+// word, 128 bytes in registers): it takes the 2-bit choices of its diagonal in the chunk's eight columns at once,
+// makes every diagonal step up to the next gap or the chunk's edge in one go, makes the gap, and takes the choices
+// again for the new diagonal, until it leaves the chunk. Between rounds every lane moves to its next chunk; the two
+// chunks a diagonal run reaches next (the same word, 8 and 16 columns back) are loaded a round and two rounds
+// ahead. The moves go to the slot's tape; the CIGAR text is made from it afterwards.
 __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
-    __shared__ ulonglong2 held[256][8];
     const int t = blockIdx.x * 256 + threadIdx.x;
-    if ((t & ~63) >= slots) return;  // whole waves only: publishing takes every lane of a wave
+    if ((t & ~63) >= slots) return;  // whole waves only: the rounds are the wave's
     const int slot = t < slots && a.walk_ready[t] ? t : -1;
     if (slot >= 0) a.walk_ready[slot] = 0;
-    // A wave walking a long pair issues first: its steps, one after another, are the walk's longest chain.
+    // A wave walking a long pair issues first: its rounds, one after another, are the walk's longest chain.
     {
         int letters = slot >= 0 ? a.s.tape.rows[slot] + a.s.tape.cols[slot] : 0;
 #pragma unroll
@@ -478,57 +506,143 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         else if (letters > 2000)
             __builtin_amdgcn_s_setprio(1);
     }
-    int64_t moves = -1;
+    OpTape ops{};
+    int r = 0, c = 0, i = 0, width = 0, words = 0;
+    const uint8_t *base = a.arena;
+    uint32_t unequal_marks = 0;  // without eqx two unequal letters are 'M' like two equal ones
+    bool outside = false;
     if (slot >= 0) {
-        const int n = a.s.tape.rows[slot], m = a.s.tape.cols[slot];
+        r = a.s.tape.rows[slot];
+        c = a.s.tape.cols[slot];
         const int bin = a.s.walk_class[slot];
-        const int hi = a.s.walk_low[slot] + edit_width(bin) - 1;
-        const uint8_t *base = a.arena + (n > 0 && m > 0 ? a.trace_offset[slot] : 0);
-        OpTape ops = op_tape(a.s, slot);
-        // The move each choice makes, 2 bits each: choice 0 (two unequal letters) 'X' (or '=' shared with 'M'
-        // without eqx), 1 'D', 2 'I', 3 two equal letters.
-        const uint32_t move_of = (a.s.eqx ? 1u : 0u) | (2u << 2) | (3u << 4);
-        const int width = edit_width(bin);
-        const int words = edit_lanes(bin) * edit_words(bin);
-        ulonglong2 *mine = held[threadIdx.x];
-        int held_chunk = -1;
-        // The chunk a diagonal run reaches next (the same word, eight columns back), loaded into registers as soon
-        // as the walk enters the one before it, and written to LDS only when the walk gets there.
-        int next_chunk = -1;
-        ulonglong2 n0{}, n1{}, n2{}, n3{}, n4{}, n5{}, n6{}, n7{};
-        auto fetch = [&](int chunk) {
-            const ulonglong2 *at = reinterpret_cast<const ulonglong2 *>(base + (size_t)chunk * 128);
-            n0 = at[0], n1 = at[1], n2 = at[2], n3 = at[3], n4 = at[4], n5 = at[5], n6 = at[6], n7 = at[7];
-        };
-        int r = n, c = m;
-        while (r > 0 && c > 0) {
-            const int i = hi - c + r;
-            if (i < 0 || i >= width) {  // a proven band holds every step; this pair's did not
-                a.s.status[slot] = 3;
-                break;
+        width = edit_width(bin);
+        words = edit_lanes(bin) * edit_words(bin);
+        i = a.s.walk_low[slot] + width - 1 - c + r;  // bit of the band's window: row r's diagonal c - r
+        if (r > 0 && c > 0) base += a.trace_offset[slot];
+        ops = op_tape(a.s, slot);
+        unequal_marks = a.s.eqx ? 0x5555u : 0u;
+    }
+    bool live = slot >= 0 && r > 0 && c > 0;
+    if (live && (i < 0 || i >= width)) {  // a proven band holds every step; this pair's did not
+        a.s.status[slot] = 3;
+        outside = true;
+        live = false;
+    }
+    // Four chunks in registers, their roles turning each round (a copy between registers would wait for every load
+    // in flight). Each round every lane loads exactly one chunk, the one it expects to walk four rounds on, so each
+    // round waits only for the load made three rounds before. A lane whose chunk is not among them (its diagonal
+    // moved into another word) loads it and sits out the rounds until it comes.
+    ulonglong2 b0[8], b1[8], b2[8], b3[8];
+    int t0 = -1, t1 = -1, t2 = -1, t3 = -1;  // the chunk each holds (-1: none)
+    int need = 0;                            // the chunk the lane walks next
+    const uint8_t *const blank = reinterpret_cast<const uint8_t *>(walk_blank);
+    if (live) need = ((c - 1) >> 3) * words + (i >> 6);
+    {
+        const auto at = [&](int chunk) { return chunk >= 0 ? base + (size_t)chunk * 128 : blank; };
+        t0 = live ? need : -1;
+        t1 = live && need >= words ? need - words : -1;
+        t2 = live && need >= 2 * words ? need - 2 * words : -1;
+        t3 = live && need >= 3 * words ? need - 3 * words : -1;
+        load_chunk(b0, at(t0));
+        load_chunk(b1, at(t1));
+        load_chunk(b2, at(t2));
+        load_chunk(b3, at(t3));
+    }
+    // One round walking `cur` (the chunk tagged `tag`); then the load for four rounds on goes into `cur`, the next
+    // three rounds walking `n1`, `n2` and `n3` in turn.
+    auto round = [&](ulonglong2 (&cur)[8], int &tag, const int &n1, const int &n2, const int &n3) -> bool {
+        if (!__any(live)) return false;
+        if (live && tag == need) {
+            const int q8 = (c - 1) >> 3, word = i >> 6;
+            for (;;) {
+                const uint32_t choices = diagonal_choices(cur, i & 63);
+                const int q = (c - 1) & 7;
+                // Gaps: choice 1 ('D') or 2 ('I'). The nearest at or before column q stops the diagonal run.
+                const uint32_t gaps = (choices ^ (choices >> 1)) & 0x5555u & ((4u << (2 * q)) - 1u);
+                const int stop = gaps ? (31 - __builtin_clz(gaps)) >> 1 : -1;
+                const int steps = q - stop < r ? q - stop : r;
+                if (steps > 0) {
+                    // Choice 0 (two unequal letters) is 'X' with eqx, choice 3 '='; column q walks first.
+                    const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
+                    const uint32_t codes = (__builtin_bitreverse32(unequal) >> 17) >> (2 * (7 - q));
+                    ops.put_many(codes & ((1u << (2 * steps)) - 1u), steps);
+                    r -= steps;
+                    c -= steps;
+                }
+                if (r == 0 || c == 0) {
+                    live = false;
+                    break;
+                }
+                if (((c - 1) >> 3) != q8) break;  // past the chunk's left edge
+                const uint32_t choice = (choices >> (2 * stop)) & 3u;
+                ops.put(choice + 1u);  // 'D' (code 2) or 'I' (code 3)
+                if (choice == 1u) {
+                    --r;
+                    --i;
+                } else {
+                    --c;
+                    ++i;
+                }
+                if (r == 0 || c == 0) {
+                    live = false;
+                    break;
+                }
+                if (i < 0 || i >= width) {
+                    a.s.status[slot] = 3;
+                    outside = true;
+                    live = false;
+                    break;
+                }
+                if ((i >> 6) != word || ((c - 1) >> 3) != q8) break;
             }
-            const int want = ((c - 1) >> 3) * words + (i >> 6);
-            if (want != held_chunk) {
-                if (want != next_chunk) fetch(want);
-                mine[0] = n0, mine[1] = n1, mine[2] = n2, mine[3] = n3;
-                mine[4] = n4, mine[5] = n5, mine[6] = n6, mine[7] = n7;
-                held_chunk = want;
-                next_chunk = want - words;
-                if (next_chunk >= 0) fetch(next_chunk);
-            }
-            const ulonglong2 cell = mine[(c - 1) & 7];
-            const uint32_t bit = i & 63;
-            const uint32_t choice = (uint32_t)((cell.x >> bit) & 1) | ((uint32_t)((cell.y >> bit) & 1) << 1);
-            ops.put((move_of >> (2 * choice)) & 3u);
-            r -= choice != 2u;
-            c -= choice != 1u;
+            if (live) need = ((c - 1) >> 3) * words + (i >> 6);
         }
+        // The chunk four rounds on: the next three rounds walk on down the diagonal's word where their chunks are
+        // the ones the lane needs by then, and wait where they are not.
+        int ahead = need;
+        ahead = n1 == ahead ? ahead - words : ahead;
+        ahead = n2 == ahead ? ahead - words : ahead;
+        ahead = n3 == ahead ? ahead - words : ahead;
+        tag = live && ahead >= 0 ? ahead : -1;
+        load_chunk(cur, tag >= 0 ? base + (size_t)tag * 128 : blank);
+        return true;
+    };
+    while (round(b0, t0, t1, t2, t3) && round(b1, t1, t2, t3, t0) && round(b2, t2, t3, t0, t1) &&
+           round(b3, t3, t0, t1, t2)) {
+    }
+    if (slot >= 0 && !outside) {
         for (; r > 0; --r) ops.put(2u);
         for (; c > 0; --c) ops.put(3u);
         ops.close();
-        moves = ops.count;
     }
-    publish_cigars(a.s, slot >= 0 ? slot : 0, moves);
+    if (t < slots) a.walk_moves[t] = slot >= 0 && !outside ? (int32_t)ops.count : -1;
+}
+
+// The CIGAR text of the pairs the walk just made moves for: a wave a pair, all 64 lanes on its tape, four pairs a
+// workgroup, their room in the output claimed with one atomic.
+__global__ __launch_bounds__(256) void edit_text(EditArgs a, int slots) {
+    __shared__ unsigned long long room[4];
+    __shared__ unsigned long long start;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * 4 + wave;
+    const int moves = slot < slots ? a.walk_moves[slot] : -1;
+    const uint64_t *words = moves >= 0 ? op_tape(a.s, slot).words : nullptr;
+    const int64_t length = moves >= 0 ? wave_cigar_text(a.s.eqx, a.s.forward_cigar, words, moves, nullptr) : 0;
+    if (lane == 0) room[wave] = (unsigned long long)length;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long total = room[0] + room[1] + room[2] + room[3];
+        start = total ? atomicAdd(a.s.cigar_used, total) : 0ull;
+    }
+    __syncthreads();
+    if (moves < 0) return;
+    unsigned long long at = start;
+    for (int w = 0; w < wave; ++w) at += room[w];
+    wave_cigar_text(a.s.eqx, a.s.forward_cigar, words, moves, a.s.cigars + at);
+    if (lane == 0) {
+        a.s.cigar_offset[slot] = (int64_t)at;
+        a.s.cigar_length[slot] = (int32_t)length;
+    }
 }
 
 // ------------------------------------------------------------------ lists
@@ -591,11 +705,12 @@ __global__ void edit_requeue(EditArgs a, const int32_t *held, int count) {
 }
 
 // Each slot's first band from the network's prediction of its cost (log(1 + cost)): the guess e^(prediction +
-// margin) - 1, and the band out from the start and end diagonals to the last diagonal a path of the guess's cost
+// margin + slope ln(max(longer side, 1000) / 1000)) - 1 (a long pair's failed band costs a long re-sweep), and the band out from the start and end diagonals to the last diagonal a path of the guess's cost
 // could visit (unit costs: a path through diagonal k > max(0, end) costs at least 2k - end, through k < min(0, end)
 // at least end - 2k), one more either side; then the slot is filed in its bin's first queue. A workgroup's slots
 // (consecutive, so of like length) take one place in each queue with one atomic a bin.
-__global__ void edit_seed(EditArgs a, const float *prediction, double margin, EditStart start, int slots) {
+__global__ void edit_seed(EditArgs a, const float *prediction, double margin, double slope, EditStart start,
+                          int slots) {
     __shared__ int local[EDIT_BINS], base[EDIT_BINS];
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (threadIdx.x < EDIT_BINS) local[threadIdx.x] = 0;
@@ -603,7 +718,9 @@ __global__ void edit_seed(EditArgs a, const float *prediction, double margin, Ed
     int bin = -1, rank = 0, low = 0;
     if (t < slots) {
         const int n = a.s.tape.rows[t], m = a.s.tape.cols[t], end = m - n;
-        const double guessed = expm1((double)prediction[t] + margin);
+        const double longer_side = n > m ? n : m;
+        const double guessed =
+            expm1((double)prediction[t] + margin + slope * log((longer_side > 1000 ? longer_side : 1000) / 1000.0));
         const long long g = guessed > 0 ? (long long)guessed : 0;
         const long long from_up = end > 0 ? end : 0, from_down = end < 0 ? end : 0;
         long long up = (g + end) >= 0 ? (g + end) / 2 : -((-(g + end) + 1) / 2);  // floor((g + end) / 2)
@@ -676,16 +793,18 @@ void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStre
         sweep_bin<SWEEP>(a, bin, grid, stream);
 }
 
-void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, const EditStart &start, int slots,
-                      hipStream_t stream) {
+void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, double slope, const EditStart &start,
+                      int slots, hipStream_t stream) {
     hipLaunchKernelGGL(edit_seed_clear, dim3(1), dim3(64), 0, stream, a.first_counts_out);
     if (slots > 0)
-        hipLaunchKernelGGL(edit_seed, dim3((slots + 255) / 256), dim3(256), 0, stream, a, prediction, margin, start,
-                           slots);
+        hipLaunchKernelGGL(edit_seed, dim3((slots + 255) / 256), dim3(256), 0, stream, a, prediction, margin, slope,
+                           start, slots);
 }
 
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {
+    if (slots <= 0) return;
     hipLaunchKernelGGL(edit_walk, dim3((slots + 255) / 256), dim3(256), 0, stream, a, slots);
+    hipLaunchKernelGGL(edit_text, dim3((slots + 3) / 4), dim3(256), 0, stream, a, slots);
 }
 
 void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream) {

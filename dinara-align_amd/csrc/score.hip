@@ -286,12 +286,7 @@ __global__ __launch_bounds__(256) void score_walk(ScoreArgs a) {
         moves = ops.count;
     }
     const uint64_t *words = reinterpret_cast<const uint64_t *>(a.op_scratch + (moves >= 0 ? a.op_base[slot] : 0));
-    const int64_t length = moves >= 0 ? cigar_text(a.eqx, 0, words, moves, nullptr) : 0;
-    const unsigned long long at = wave_claim(a.cigar_used, (unsigned long long)length);
-    if (moves < 0) return;
-    cigar_text(a.eqx, 0, words, moves, a.cigars + at);
-    a.cigar_offset[slot] = (int64_t)at;
-    a.cigar_length[slot] = (int32_t)length;
+    publish_wave(a.eqx, 0, words, moves, a.cigars, a.cigar_used, a.cigar_offset + slot, a.cigar_length + slot);
 }
 
 // ------------------------------------------------------------------ launches

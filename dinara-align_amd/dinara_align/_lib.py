@@ -122,6 +122,8 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_seed.argtypes = [vp, vp]
     handle.dinara_batch_seed.restype = i64
     handle.dinara_batch_seed_learned.argtypes = [vp, vp, ctypes.c_double]
+    handle.dinara_batch_seed_learned_by_length.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
+    handle.dinara_batch_seed_learned_by_length.restype = i64
     handle.dinara_batch_seed_learned.restype = i64
     handle.dinara_batch_set_retry.argtypes = [vp, i64]
     handle.dinara_batch_set_retry.restype = i64

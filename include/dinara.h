@@ -232,6 +232,9 @@ typedef struct dinara_mlp dinara_mlp;
 /* The same seeds from a trained network without leaving the device: the batch's features, the network's
  * predictions, and each pair's guess e^(prediction + margin) - 1. */
 int64_t dinara_batch_seed_learned(dinara_batch *batch, dinara_mlp *net, double margin);
+/* The same with a margin that grows with the pair's longer side: e^(prediction + margin + slope ln(max(longer, 1000)
+ * / 1000)) - 1, since a long pair whose first band fails costs a long second sweep. */
+int64_t dinara_batch_seed_learned_by_length(dinara_batch *batch, dinara_mlp *net, double margin, double slope);
 
 /* How an unproven pair's next band is chosen: 0 (the default) the band its found cost proves, 1 plain band doubling,
  * the same band twice as wide (the policy the learned band is measured against). */

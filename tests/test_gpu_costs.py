@@ -338,3 +338,16 @@ def test_c_abi_codes_per_pair():
     assert found[1].cigar is None
     for item in found:
         handle.dinara_free(item.cigar)
+
+
+def test_cigar_runs_across_chunk_edges():
+    # Runs longer than 2048 moves and runs ending at move 2048 exactly: the CIGAR text is made 2048 moves at a time.
+    rng = random.Random(59)
+    base = "".join(rng.choice("ACGT") for _ in range(5200))
+    other = {"A": "C", "C": "G", "G": "T", "T": "A"}
+    refs = [base[:5000], base[:4100], base[:4096], base[:3000]]
+    qrys = [base[:5000], base[:2047] + other[base[2047]] + base[2048:4100], base[:2047] + base[2048:4096],
+            base[:3000] + base[4000:5100]]
+    for costs in (da.Costs.affine(4, 6, 2), da.Costs.linear(3, 2)):
+        check_alignments(refs, qrys, costs)
+        check_alignments(refs, qrys, costs, ties="right", eqx=False)

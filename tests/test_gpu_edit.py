@@ -343,3 +343,37 @@ def test_learned_seeds_on_the_device_change_no_result():
                     b.free()
         finally:
             h.dinara_mlp_free(net)
+
+
+def chunk_edge_pairs():
+    """Pairs whose CIGARs hold runs longer than 2048 moves and runs ending at move 2048 or 4096 exactly: the
+    CIGAR text is made 2048 moves at a time, a run carried from one piece to the next."""
+    rng = random.Random(57)
+    base = "".join(rng.choice("ACGT") for _ in range(6200))
+    other = {"A": "C", "C": "G", "G": "T", "T": "A"}
+    refs, qrys = [], []
+    refs.append(base[:5000]); qrys.append(base[:5000])                                        # one run of 5000
+    refs.append(base[:4100]); qrys.append(base[:2048] + other[base[2048]] + base[2049:4100])   # X at 2048
+    refs.append(base[:4100]); qrys.append(base[:2047] + other[base[2047]] + base[2048:4100])   # X ends at 2048
+    refs.append(base[:4096]); qrys.append(base[:2047] + base[2048:4096])                       # D at 2047
+    refs.append(base[:4097]); qrys.append(base[:4095] + other[base[4095]] + base[4096])        # X ends at 4096
+    refs.append(base[:3000]); qrys.append(base[:3000] + base[3100:6100])                      # 3000 I
+    refs.append(base[:2100]); qrys.append(base[:20])                                           # 2080 D
+    refs.append(base[:2048]); qrys.append(base[:2048])                                         # exactly 2048
+    return refs, qrys
+
+
+@pytest.mark.parametrize("right", [False, True])
+@pytest.mark.parametrize("eqx", [True, False])
+def test_cigar_runs_across_chunk_edges(right, eqx):
+    refs, qrys = chunk_edge_pairs()
+    b = Batch(refs, qrys, 1, ties_right=right, eqx=eqx)
+    try:
+        got = b.run()
+    finally:
+        b.free()
+    assert got == oracle_alignments(refs, qrys, right=right, eqx=eqx)
+    if eqx:
+        assert got[0] == (0, "5000=") and got[1][1] == "2048=1X2051=" and got[7] == (0, "2048=")
+    else:
+        assert got[0] == (0, "5000M") and got[1][1] == "4100M"

@@ -178,3 +178,14 @@ def test_local_protein_workload_against_oracle():
     scoring = da.Scoring.tabulated(alphabet, table, -11, -1)
     refs, qrys = synthetic.local_protein(24, seed=7, query=300, window=3000)
     check_local([r.decode() for r in refs], [q.decode() for q in qrys], scoring)
+
+
+def test_cigar_runs_across_chunk_edges():
+    # Runs longer than 2048 moves and a run ending at move 2048 exactly: the CIGAR text is made 2048 moves at a time.
+    rng = random.Random(15)
+    base = "".join(rng.choice("ACGT") for _ in range(4300))
+    other = {"A": "C", "C": "G", "G": "T", "T": "A"}
+    refs = [base[:4300], base[:4100]]
+    qrys = [base[:4300], base[:2047] + other[base[2047]] + base[2048:4100]]
+    check_global(refs, qrys, DNA)
+    check_global(refs, qrys, DNA, eqx=False)
