@@ -21,7 +21,7 @@ struct OpTape {
             held = 0;
         }
     }
-    // `moves` (at most 8) moves at once, the first in bits 0-1 of `codes`.
+    // `moves` (1 to 16) moves at once, the first in bits 0-1 of `codes`.
     __device__ void put_many(uint32_t codes, int moves) {
         const int fill = (int)(count & 31);
         held |= (uint64_t)codes << (2 * fill);

@@ -462,8 +462,8 @@ __device__ __forceinline__ uint32_t diagonal_choices(const ulonglong2 (&chunk)[8
     for (int q = 0; q < 8; ++q) {
         const uint32_t x = high ? (uint32_t)(chunk[q].x >> 32) : (uint32_t)chunk[q].x;
         const uint32_t y = high ? (uint32_t)(chunk[q].y >> 32) : (uint32_t)chunk[q].y;
-        packed |= ((x >> shift) & 1u) << (2 * q);
-        packed |= ((y >> shift) & 1u) << (2 * q + 1);
+        packed |= __builtin_amdgcn_ubfe(x, shift, 1u) << (2 * q);
+        packed |= __builtin_amdgcn_ubfe(y, shift, 1u) << (2 * q + 1);
     }
     return packed;
 }
@@ -473,6 +473,9 @@ __device__ __forceinline__ void load_chunk(ulonglong2 (&to)[8], const uint8_t *f
 #pragma unroll
     for (int q = 0; q < 8; ++q) to[q] = at[q];
 }
+
+// s_waitcnt's operand that waits for every vector memory access (vmcnt 0) and nothing else (expcnt 7, lgkmcnt 15).
+constexpr int WAIT_FOR_VECTOR_MEMORY = (7 << 4) | (15 << 8);
 
 // What a lane with no chunk to walk loads, so that every lane makes the same loads.
 __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (one address space, global loads)
@@ -547,53 +550,50 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         load_chunk(b1, at(t1));
         load_chunk(b2, at(t2));
         load_chunk(b3, at(t3));
+        // Once, before the rounds: with nothing in flight at their start, each round's wait counts only the loads
+        // of the rounds since its chunk's (the four loads above may be issued in any order).
+        __builtin_amdgcn_s_waitcnt(WAIT_FOR_VECTOR_MEMORY);
     }
     // One round walking `cur` (the chunk tagged `tag`); then the load for four rounds on goes into `cur`, the next
     // three rounds walking `n1`, `n2` and `n3` in turn.
     auto round = [&](ulonglong2 (&cur)[8], int &tag, const int &n1, const int &n2, const int &n3) -> bool {
         if (!__any(live)) return false;
+        // Every lane reads its chunk, walking or not (the empty asm keeps the read out of the branch below): the
+        // chunk's load is then waited for on every path, and its registers are free for this round's load.
+        uint32_t choices = diagonal_choices(cur, i & 63);
+        asm volatile("" : : "v"(choices));
         if (live && tag == need) {
             const int q8 = (c - 1) >> 3, word = i >> 6;
             for (;;) {
-                const uint32_t choices = diagonal_choices(cur, i & 63);
+                // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the chunk's edge,
+                // then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
                 const int q = (c - 1) & 7;
-                // Gaps: choice 1 ('D') or 2 ('I'). The nearest at or before column q stops the diagonal run.
                 const uint32_t gaps = (choices ^ (choices >> 1)) & 0x5555u & ((4u << (2 * q)) - 1u);
                 const int stop = gaps ? (31 - __builtin_clz(gaps)) >> 1 : -1;
                 const int steps = q - stop < r ? q - stop : r;
-                if (steps > 0) {
-                    // Choice 0 (two unequal letters) is 'X' with eqx, choice 3 '='; column q walks first.
-                    const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
-                    const uint32_t codes = (__builtin_bitreverse32(unequal) >> 17) >> (2 * (7 - q));
-                    ops.put_many(codes & ((1u << (2 * steps)) - 1u), steps);
-                    r -= steps;
-                    c -= steps;
-                }
-                if (r == 0 || c == 0) {
-                    live = false;
+                const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
+                uint32_t codes = ((__builtin_bitreverse32(unequal) >> 17) >> (2 * (7 - q))) & ((1u << (2 * steps)) - 1u);
+                r -= steps;
+                c -= steps;
+                const bool gap = stop >= 0 && r > 0 && c > 0;
+                const uint32_t choice = (choices >> (2 * (stop & 7))) & 3u;
+                codes |= gap ? (choice + 1u) << (2 * steps) : 0u;  // 'D' (code 2) or 'I' (code 3)
+                ops.put_many(codes, steps + (gap ? 1 : 0));
+                const int deletion = gap && choice == 1u, insertion = gap && choice != 1u;
+                r -= deletion;
+                c -= insertion;
+                i += insertion - deletion;
+                const bool finished = r == 0 || c == 0;
+                const bool astray = !finished && (i < 0 || i >= width);
+                if (finished || astray || (i >> 6) != word || ((c - 1) >> 3) != q8) {
+                    if (astray) {  // a proven band holds every step; this pair's did not
+                        a.s.status[slot] = 3;
+                        outside = true;
+                    }
+                    live = !(finished || astray);
                     break;
                 }
-                if (((c - 1) >> 3) != q8) break;  // past the chunk's left edge
-                const uint32_t choice = (choices >> (2 * stop)) & 3u;
-                ops.put(choice + 1u);  // 'D' (code 2) or 'I' (code 3)
-                if (choice == 1u) {
-                    --r;
-                    --i;
-                } else {
-                    --c;
-                    ++i;
-                }
-                if (r == 0 || c == 0) {
-                    live = false;
-                    break;
-                }
-                if (i < 0 || i >= width) {
-                    a.s.status[slot] = 3;
-                    outside = true;
-                    live = false;
-                    break;
-                }
-                if ((i >> 6) != word || ((c - 1) >> 3) != q8) break;
+                choices = diagonal_choices(cur, i & 63);
             }
             if (live) need = ((c - 1) >> 3) * words + (i >> 6);
         }
