@@ -873,7 +873,8 @@ def main() -> int:
     ap.add_argument("--margin", type=float, default=0.0,
                     help="uneven: learned seeds e^(prediction + margin + slope ln(max(longer, 1000) / 1000)) - 1")
     ap.add_argument("--slope", type=float, default=0.0, help="uneven: the seeds' margin per e-fold of length")
-    ap.add_argument("--slopes", default="", help="uneven: comma-separated slopes to time, keeping the fastest")
+    ap.add_argument("--slopes", default="0,0.1,0.2",
+                    help="uneven: comma-separated slopes to time on the batch, keeping the fastest ('' for --slope)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "
