@@ -669,12 +669,17 @@ def run_uneven(args) -> int:
         learned_chosen = (ctypes.c_int64 * 7)()
         if handle.dinara_batch_autotune(batch, learned_chosen) != 0:
             raise RuntimeError("autotune failed")
+        # The seeds' margin and slope, timed on the batch (like the launch shape): the fastest pair is kept.
         by_slope = {}
-        if args.slopes:
-            for slope in [float(v) for v in args.slopes.split(",")]:
-                by_slope[slope] = timed_steps(handle, batch, 2, 1, before=seed)[0]
-                print(f"[bench] slope {slope}: {by_slope[slope]:.1f} ms", file=sys.stderr, flush=True)
-            slope = min(by_slope, key=by_slope.get)
+        if args.slopes or args.margins:
+            slopes = [float(v) for v in args.slopes.split(",")] if args.slopes else [args.slope]
+            margins = [float(v) for v in args.margins.split(",")] if args.margins else [args.margin]
+            for margin in margins:
+                for slope in slopes:
+                    by_slope[(margin, slope)] = timed_steps(handle, batch, 2, 1, before=seed)[0]
+                    print(f"[bench] margin {margin} slope {slope}: {by_slope[(margin, slope)]:.1f} ms",
+                          file=sys.stderr, flush=True)
+            margin, slope = min(by_slope, key=by_slope.get)
         learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
         results["learned"] = edit_results(handle, batch, A, pairs)
         learned_proven = handle.dinara_batch_stat(batch, 4)
@@ -682,7 +687,8 @@ def run_uneven(args) -> int:
         learned_prof = bin_profile(handle, batch)
         learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
                         "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
-                        "margin": margin, "slope": slope, "by_slope_ms": {str(k): round(v, 3) for k, v in by_slope.items()},
+                        "margin": margin, "slope": slope,
+                        "by_margin_slope_ms": {f"{k[0]},{k[1]}": round(v, 3) for k, v in by_slope.items()},
                         "first_band_proved": round(learned_proven / pairs, 4),
                         "walked_from_first_sweep": round(walked_first / pairs, 4),
                         "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
@@ -875,6 +881,8 @@ def main() -> int:
     ap.add_argument("--slope", type=float, default=0.0, help="uneven: the seeds' margin per e-fold of length")
     ap.add_argument("--slopes", default="0,0.1,0.2",
                     help="uneven: comma-separated slopes to time on the batch, keeping the fastest ('' for --slope)")
+    ap.add_argument("--margins", default="-0.1,0,0.1",
+                    help="uneven: comma-separated margins timed with each slope ('' for --margin)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "

@@ -28,8 +28,13 @@ namespace dinara {
 
 // ------------------------------------------------------------------ features
 
+// 4-mer counts of 64 slots a workgroup, a thread a slot, in LDS (the thread's own column, so no thread waits on
+// another). A 2-bit tape is read a word (16 letters) at a time: the 16 windows that end in the word come from the
+// word and the one before it as 64 bits, and each is counted with an LDS add whose result is not waited for. The
+// windows are counted by their tape bits (first letter lowest); a forward 4-mer code puts the first letter highest,
+// so a forward tape's counts are read back with the four 2-bit fields reversed.
 __global__ __launch_bounds__(64) void kmer_features(Tape tape, int slots, int reversed, float *out) {
-    __shared__ int16_t hist[256 * 64];
+    __shared__ int32_t hist[256 * 64];
     const int t = threadIdx.x, slot = blockIdx.x * 64 + t;
     for (int k = 0; k < 256; ++k) hist[k * 64 + t] = 0;
     if (slot >= slots) return;
@@ -38,26 +43,58 @@ __global__ __launch_bounds__(64) void kmer_features(Tape tape, int slots, int re
     const uint32_t *qry_words = ref_words + (size_t)tape.group_ref_words[g] * 64;
     const int bits = tape.bits, per = 32 / bits;
     const int n = tape.rows[slot], m = tape.cols[slot];
-    // Every 4-mer of a sequence, as its forward code (a reversed tape shifts each letter in from the top).
     auto count = [&](const uint32_t *words, int len, int delta) {
-        uint32_t code = 0;
+        if (bits == 2) {
+            // Eight words a step, the next eight loaded while these are counted.
+            const int total = (len + 15) >> 4;
+            auto load = [&](uint32_t (&to)[8], int w0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) to[k] = w0 + k < total ? words[(size_t)(w0 + k) * 64] : 0u;
+            };
+            uint32_t now[8], next[8];
+            load(now, 0);
+            uint32_t before = 0;
+            for (int w0 = 0; w0 < total; w0 += 8) {
+                load(next, w0 + 8);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int w = w0 + k;
+                    const uint64_t pair = (uint64_t)before | ((uint64_t)now[k] << 32);
+                    // Window j starts at letter 16w + j - 3 and ends at letter 16w + j.
+                    const int first = w == 0 ? 3 : 0, last = len - 16 * w;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        if (j >= first && j < last)
+                            atomicAdd(&hist[(uint32_t)((pair >> (26 + 2 * j)) & 255u) * 64 + t], delta);
+                    }
+                    before = now[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) now[k] = next[k];
+            }
+            return;
+        }
+        uint32_t window = 0;  // the last four letters, first lowest
         int seen = 0;
         for (int w = 0; w * per < len; ++w) {
             uint32_t word = words[(size_t)w * 64];
             const int here = min(per, len - w * per);
             for (int i = 0; i < here; ++i) {
-                const uint32_t letter = word & 3u;
+                window = (window >> 2) | ((word & 3u) << 6);
                 word >>= bits;
-                code = reversed ? ((code >> 2) | (letter << 6)) : (((code << 2) | letter) & 255u);
-                if (++seen >= 4) hist[code * 64 + t] = (int16_t)(hist[code * 64 + t] + delta);
+                if (++seen >= 4) atomicAdd(&hist[window * 64 + t], delta);
             }
         }
     };
     count(ref_words, n, 1);
     count(qry_words, m, -1);
+    // A reversed tape holds each sequence back to front: its windows' tape bits are then the forward codes.
     float *row = out + (size_t)slot * FEATURE_STRIDE;
     const float scale = 8.0f / sqrtf(1.0f + (float)n + (float)m);
-    for (int k = 0; k < 256; ++k) row[k] = (float)hist[k * 64 + t] * scale;
+    for (int k = 0; k < 256; ++k) {
+        const int at = reversed ? k : ((k & 3) << 6) | ((k & 12) << 2) | ((k & 48) >> 2) | (k >> 6);
+        row[k] = (float)hist[at * 64 + t] * scale;
+    }
     row[256] = (logf(1.0f + (float)n) - 7.0f) * 0.5f;
     row[257] = (logf(1.0f + (float)m) - 7.0f) * 0.5f;
     for (int k = FEATURES; k < FEATURE_STRIDE; ++k) row[k] = 0.0f;
@@ -122,6 +159,80 @@ static void gemm(hipStream_t stream, const uint16_t *a, int lda, const uint16_t 
                  int ldcb) {
     hipLaunchKernelGGL((gemm_bf16<TA, TB>), dim3(M / 32, N / 32), dim3(64), 0, stream, a, lda, b, ldb, K, bias,
                        relu, mask, ldm, c, ldc, cb, ldcb);
+}
+
+// ------------------------------------------------------------------ prediction in one pass
+
+// (rows x cols) bf16 into (cols x rows).
+__global__ void transpose_bf16(const uint16_t *in, int rows, int cols, uint16_t *out) {
+    const int at = blockIdx.x * blockDim.x + threadIdx.x;
+    if (at >= rows * cols) return;
+    out[(size_t)(at % cols) * rows + at / cols] = in[at];
+}
+
+constexpr int PREDICT_HIDDEN = 256;
+
+// The network's prediction for 32 rows a wave, both hidden layers and the output in one pass: the fp32 features
+// are rounded to bf16 as they are read, each layer is 8 MFMA tiles of 32 x 32 (the weights read transposed, 4
+// bf16 a lane), and a layer's activations go through LDS to become the next product's left operand. The products
+// take their K steps in the order `gemm_bf16` takes them, so the prediction is the same as the layer-by-layer one.
+__global__ __launch_bounds__(64) void predict_rows(const float *features, int rows, const uint16_t *w1t,
+                                                   const float *b1, const uint16_t *w2t, const float *b2,
+                                                   const uint16_t *w3, const float *b3, float *y) {
+    __shared__ uint16_t act[32 * PREDICT_HIDDEN];
+    const int lane = threadIdx.x, row = lane & 31, kq = (lane >> 5) * 4;
+    const int i0 = blockIdx.x * 32;
+    f32x16 acc[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) acc[nt] = f32x16{};
+    const bool real = i0 + row < rows;
+    const float *x = features + (size_t)(real ? i0 + row : 0) * FEATURE_STRIDE;
+    for (int k0 = 0; k0 < FEATURE_STRIDE; k0 += 8) {
+        const float4 f = real ? *reinterpret_cast<const float4 *>(x + k0 + kq) : make_float4(0.f, 0.f, 0.f, 0.f);
+        bf16x4 va;
+        va[0] = (short)to_bf16(f.x);
+        va[1] = (short)to_bf16(f.y);
+        va[2] = (short)to_bf16(f.z);
+        va[3] = (short)to_bf16(f.w);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const bf16x4 vb = *reinterpret_cast<const bf16x4 *>(w1t + (size_t)(nt * 32 + row) * FEATURE_STRIDE + k0 + kq);
+            acc[nt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb, acc[nt], 0, 0, 0);
+        }
+    }
+    // Bias, ReLU and bf16 into LDS, row-major: MFMA result r of lane l is row 8(r/4) + 4(l/32) + r%4, column l%32.
+    auto keep = [&](const float *bias) {
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const int j = nt * 32 + row;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+                const float v = acc[nt][r] + bias[j];
+                act[i * PREDICT_HIDDEN + j] = to_bf16(v > 0.0f ? v : 0.0f);
+            }
+            acc[nt] = f32x16{};
+        }
+    };
+    keep(b1);
+    __syncthreads();
+    for (int k0 = 0; k0 < PREDICT_HIDDEN; k0 += 8) {
+        const bf16x4 va = *reinterpret_cast<const bf16x4 *>(act + row * PREDICT_HIDDEN + k0 + kq);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const bf16x4 vb = *reinterpret_cast<const bf16x4 *>(w2t + (size_t)(nt * 32 + row) * PREDICT_HIDDEN + k0 + kq);
+            acc[nt] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(va, vb, acc[nt], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    keep(b2);
+    __syncthreads();
+    // The output: one lane a row, summed in column order like `head`.
+    if (lane < 32 && real) {
+        float sum = b3[0];
+        for (int k = 0; k < PREDICT_HIDDEN; ++k) sum += from_bf16(act[row * PREDICT_HIDDEN + k]) * from_bf16(w3[k]);
+        y[i0 + row] = sum;
+    }
 }
 
 // ------------------------------------------------------------------ the network's small kernels
@@ -203,6 +314,8 @@ struct dinara_mlp {
     // per-step work
     DeviceBuffer x, z1, a1, z2, a2, y, dy, sq, dz2, dz1, targets, index, loss, features;
     DeviceBuffer identity;  // 0, 1, 2, ... for gathers of consecutive rows
+    DeviceBuffer transposed;  // w1 and w2 as bf16, transposed, for the one-pass prediction
+    DeviceBuffer predicted;   // the one-pass prediction's output for `dinara_mlp_predict`
     int identity_rows = 0;
     size_t off_w1, off_b1, off_w2, off_b2, off_w3, off_b3, total;
 };
@@ -313,6 +426,19 @@ int64_t mlp_predict_into(dinara_mlp *net, const float *features, int64_t rows, f
     if (rows <= 0) return 0;
     DeviceGuard guard(net->ctx);
     hipStream_t s = net->ctx->stream;
+    if (net->hidden == PREDICT_HIDDEN && rows <= INT32_MAX / FEATURE_STRIDE) {
+        const size_t w1 = (size_t)FEATURE_STRIDE * PREDICT_HIDDEN, w2 = (size_t)PREDICT_HIDDEN * PREDICT_HIDDEN;
+        if (!net->transposed.reserve((w1 + w2) * 2)) return DINARA_OUT_OF_MEMORY;
+        uint16_t *w1t = h16(net->transposed), *w2t = w1t + w1;
+        hipLaunchKernelGGL(transpose_bf16, dim3((unsigned)((w1 + 255) / 256)), dim3(256), 0, s,
+                           h16(net->bf, net->off_w1), FEATURE_STRIDE, PREDICT_HIDDEN, w1t);
+        hipLaunchKernelGGL(transpose_bf16, dim3((unsigned)((w2 + 255) / 256)), dim3(256), 0, s,
+                           h16(net->bf, net->off_w2), PREDICT_HIDDEN, PREDICT_HIDDEN, w2t);
+        hipLaunchKernelGGL(predict_rows, dim3((unsigned)((rows + 31) / 32)), dim3(64), 0, s, features, (int)rows, w1t,
+                           f32(net->master, net->off_b1), w2t, f32(net->master, net->off_b2),
+                           h16(net->bf, net->off_w3), f32(net->master, net->off_b3), out);
+        return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
+    }
     const int chunk = 1 << 16;
     if (!reserve_rows(net, chunk)) return DINARA_OUT_OF_MEMORY;
     if (net->identity_rows < chunk) {
@@ -424,6 +550,14 @@ int64_t dinara_mlp_predict(dinara_mlp *net, int64_t rows, const float *features,
     if (!net) return DINARA_INVALID_MODE;
     if (rows <= 0) return 0;
     DeviceGuard guard(net->ctx);
+    if (net->hidden == PREDICT_HIDDEN && rows <= INT32_MAX / FEATURE_STRIDE) {  // the one-pass prediction
+        const size_t bytes = (size_t)rows * FEATURE_STRIDE * 4;
+        if (!net->features.reserve(bytes) || !net->features.write(net->ctx, features, bytes) ||
+            !net->predicted.reserve((size_t)rows * 4))
+            return DINARA_OUT_OF_MEMORY;
+        if (const int64_t code = mlp_predict_into(net, f32(net->features), rows, f32(net->predicted))) return code;
+        return net->predicted.read(net->ctx, out, (size_t)rows * 4) ? 0 : DINARA_OUT_OF_MEMORY;
+    }
     const int64_t chunk = 1 << 16;
     for (int64_t first = 0; first < rows; first += chunk) {
         const int64_t count = std::min(chunk, rows - first);
