@@ -21,13 +21,13 @@ struct OpTape {
             held = 0;
         }
     }
-    // `moves` (1 to 16) moves at once, the first in bits 0-1 of `codes`.
-    __device__ void put_many(uint32_t codes, int moves) {
+    // `moves` (1 to 24) moves at once, the first in bits 0-1 of `codes`.
+    __device__ void put_many(uint64_t codes, int moves) {
         const int fill = (int)(count & 31);
-        held |= (uint64_t)codes << (2 * fill);
+        held |= codes << (2 * fill);
         if (fill + moves >= 32) {
             words[count >> 5] = held;
-            held = fill + moves > 32 ? (uint64_t)codes >> (2 * (32 - fill)) : 0;
+            held = fill + moves > 32 ? codes >> (2 * (32 - fill)) : 0;
         }
         count += moves;
     }

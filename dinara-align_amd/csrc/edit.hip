@@ -452,32 +452,24 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 
 // ------------------------------------------------------------------ the walk
 
-// The 2-bit choices of one diagonal (bit `bit` of the word) in the eight columns of a chunk, column q's in bits 2q
-// and 2q + 1.
-__device__ __forceinline__ uint32_t diagonal_choices(const ulonglong2 (&chunk)[8], int bit) {
-    const bool high = bit >= 32;
-    const uint32_t shift = (uint32_t)bit & 31u;
+// The 2-bit choices of one diagonal in the sixteen columns of a span, column q's in bits 2q and 2q + 1, from the
+// 32-bit halves (of the word's two decision planes) that hold the diagonal's bit, `shift` within them.
+__device__ __forceinline__ uint32_t diagonal_choices(const uint32_t (&x)[16], const uint32_t (&y)[16], uint32_t shift) {
     uint32_t packed = 0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const uint32_t x = high ? (uint32_t)(chunk[q].x >> 32) : (uint32_t)chunk[q].x;
-        const uint32_t y = high ? (uint32_t)(chunk[q].y >> 32) : (uint32_t)chunk[q].y;
-        packed |= __builtin_amdgcn_ubfe(x, shift, 1u) << (2 * q);
-        packed |= __builtin_amdgcn_ubfe(y, shift, 1u) << (2 * q + 1);
+    for (int q = 0; q < 16; ++q) {
+        packed |= __builtin_amdgcn_ubfe(x[q], shift, 1u) << (2 * q);
+        packed |= __builtin_amdgcn_ubfe(y[q], shift, 1u) << (2 * q + 1);
     }
     return packed;
 }
 
-__device__ __forceinline__ void load_chunk(ulonglong2 (&to)[8], const uint8_t *from) {
-    const ulonglong2 *at = reinterpret_cast<const ulonglong2 *>(from);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) to[q] = at[q];
-}
+// Sixteen bytes of kept decisions in global memory, reached from an address held as an integer (one lane loads
+// for another): marked global so the load is a global load, not a flat one.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) global_chunk;
 
-// s_waitcnt's operand that waits for every vector memory access (vmcnt 0) and nothing else (expcnt 7, lgkmcnt 15).
-constexpr int WAIT_FOR_VECTOR_MEMORY = (7 << 4) | (15 << 8);
-
-// What a lane with no chunk to walk loads, so that every lane makes the same loads.
+// What a lane with no span to walk loads, so that every lane makes the same loads.
 __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (one address space, global loads)
 
 // One thread a slot, walking it if it is ready: from (n, m) back to the origin over the 2-bit decisions. Slots are
@@ -490,6 +482,8 @@ __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (o
 // chunks a diagonal run reaches next (the same word, 8 and 16 columns back) are loaded a round and two rounds
 // ahead. The moves go to the slot's tape; the CIGAR text is made from it afterwards.
 __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
+    __shared__ u32x4 span_stage[256][16];  // each lane's span, put together by the lanes that loaded it
+    __shared__ ulonglong2 span_where[256];      // each lane's span: its two chunks' addresses
     const int t = blockIdx.x * 256 + threadIdx.x;
     if ((t & ~63) >= slots) return;  // whole waves only: the rounds are the wave's
     const int slot = t < slots && a.walk_ready[t] ? t : -1;
@@ -523,7 +517,7 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         i = a.s.walk_low[slot] + width - 1 - c + r;  // bit of the band's window: row r's diagonal c - r
         if (r > 0 && c > 0) base += a.trace_offset[slot];
         ops = op_tape(a.s, slot);
-        unequal_marks = a.s.eqx ? 0x5555u : 0u;
+        unequal_marks = a.s.eqx ? 0x55555555u : 0u;
     }
     bool live = slot >= 0 && r > 0 && c > 0;
     if (live && (i < 0 || i >= width)) {  // a proven band holds every step; this pair's did not
@@ -531,53 +525,84 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         outside = true;
         live = false;
     }
-    // Four chunks in registers, their roles turning each round (a copy between registers would wait for every load
-    // in flight). Each round every lane loads exactly one chunk, the one it expects to walk four rounds on, so each
-    // round waits only for the load made three rounds before. A lane whose chunk is not among them (its diagonal
-    // moved into another word) loads it and sits out the rounds until it comes.
-    ulonglong2 b0[8], b1[8], b2[8], b3[8];
-    int t0 = -1, t1 = -1, t2 = -1, t3 = -1;  // the chunk each holds (-1: none)
-    int need = 0;                            // the chunk the lane walks next
-    const uint8_t *const blank = reinterpret_cast<const uint8_t *>(walk_blank);
-    if (live) need = ((c - 1) >> 3) * words + (i >> 6);
-    {
-        const auto at = [&](int chunk) { return chunk >= 0 ? base + (size_t)chunk * 128 : blank; };
-        t0 = live ? need : -1;
-        t1 = live && need >= words ? need - words : -1;
-        t2 = live && need >= 2 * words ? need - 2 * words : -1;
-        t3 = live && need >= 3 * words ? need - 3 * words : -1;
-        load_chunk(b0, at(t0));
-        load_chunk(b1, at(t1));
-        load_chunk(b2, at(t2));
-        load_chunk(b3, at(t3));
-        // Once, before the rounds: with nothing in flight at their start, each round's wait counts only the loads
-        // of the rounds since its chunk's (the four loads above may be issued in any order).
-        __builtin_amdgcn_s_waitcnt(WAIT_FOR_VECTOR_MEMORY);
-    }
-    // One round walking `cur` (the chunk tagged `tag`); then the load for four rounds on goes into `cur`, the next
-    // three rounds walking `n1`, `n2` and `n3` in turn.
-    auto round = [&](ulonglong2 (&cur)[8], int &tag, const int &n1, const int &n2, const int &n3) -> bool {
-        if (!__any(live)) return false;
-        // Every lane reads its chunk, walking or not (the empty asm keeps the read out of the branch below): the
-        // chunk's load is then waited for on every path, and its registers are free for this round's load.
-        uint32_t choices = diagonal_choices(cur, i & 63);
-        asm volatile("" : : "v"(choices));
-        if (live && tag == need) {
-            const int q8 = (c - 1) >> 3, word = i >> 6;
+    // A lane's span (its next sixteen columns of one word: two 128-byte chunks) is loaded by sixteen lanes of its
+    // wave, eight to a chunk, so that each load instruction reads eight whole lines rather than a piece of 64;
+    // the pieces go to the lane through LDS. A round first takes the pieces loaded for it a round before (waiting
+    // for them), then starts the loads of the span the lane expects next (the same word, sixteen columns back) and
+    // walks while they are in flight. A lane whose next span is not the one loaded (its diagonal moved into another
+    // word) sits out a round while it comes. Span s of word w is tagged s·words + w.
+    u32x4 pieces[16];
+    uint32_t half_x[16], half_y[16];  // the span's decision planes, the 32-bit half holding the lane's diagonal
+    int half = 0;
+    int now_tag = -1, loaded_tag = -1;
+    int need = 0;  // the span the lane walks next
+    const int chunks = (c + 7) >> 3;
+    const int lane = threadIdx.x & 63, wave_base = threadIdx.x & ~63;
+    const uint64_t blank = (uint64_t)(uintptr_t)walk_blank;
+    // Every lane of the wave: publish the span's two chunk addresses, then load piece `lane % 16` of the spans of
+    // lanes 4j + lane / 16, j = 0..15.
+    const auto issue = [&](int tag) __attribute__((always_inline)) {
+        uint64_t low = blank, high = blank;
+        if (tag >= 0) {
+            const int span = tag / words, word = tag - span * words;
+            const int lo = 2 * span, hi = 2 * span + 1 < chunks ? 2 * span + 1 : lo;  // past the pair: any chunk
+            low = (uint64_t)(uintptr_t)(base + ((size_t)lo * words + word) * 128);
+            high = (uint64_t)(uintptr_t)(base + ((size_t)hi * words + word) * 128);
+        }
+        span_where[threadIdx.x] = make_ulonglong2(low, high);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const ulonglong2 where = span_where[wave_base + 4 * j + (lane >> 4)];
+            const uint64_t line = (lane & 8) ? where.y : where.x;
+            pieces[j] = *reinterpret_cast<global_chunk *>(line + (uint64_t)(lane & 7) * 16);
+        }
+        loaded_tag = tag;
+    };
+    // This lane's span in LDS (column q: plane 0's 64 bits, then plane 1's): the half `h` of both planes.
+    const uint32_t *const mine = reinterpret_cast<const uint32_t *>(&span_stage[threadIdx.x][0]);
+    const auto read_half = [&](int h) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            half_x[q] = mine[4 * q + h];
+            half_y[q] = mine[4 * q + 2 + h];
+        }
+        half = h;
+    };
+    // Every lane of the wave: hand the loaded pieces to their lanes through LDS, and take this lane's span.
+    const auto take = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) span_stage[wave_base + 4 * j + (lane >> 4)][lane & 15] = pieces[j];
+        __builtin_amdgcn_wave_barrier();
+        read_half((i & 63) >> 5);
+        now_tag = loaded_tag;
+    };
+    if (live) need = ((c - 1) >> 4) * words + (i >> 6);
+    issue(live ? need : -1);
+    while (__any(live)) {
+        take();
+        uint32_t choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
+        const bool walking = live && now_tag == need;
+        const int expected = walking ? need - words : need;
+        issue(live && expected >= 0 ? expected : -1);
+        if (walking) {
+            const int span = (c - 1) >> 4, word = i >> 6;
             for (;;) {
-                // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the chunk's edge,
+                // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the span's edge,
                 // then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
-                const int q = (c - 1) & 7;
-                const uint32_t gaps = (choices ^ (choices >> 1)) & 0x5555u & ((4u << (2 * q)) - 1u);
+                const int q = (c - 1) & 15;
+                const uint32_t upto = q == 15 ? 0xFFFFFFFFu : (4u << (2 * q)) - 1u;
+                const uint32_t gaps = (choices ^ (choices >> 1)) & 0x55555555u & upto;
                 const int stop = gaps ? (31 - __builtin_clz(gaps)) >> 1 : -1;
                 const int steps = q - stop < r ? q - stop : r;
                 const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
-                uint32_t codes = ((__builtin_bitreverse32(unequal) >> 17) >> (2 * (7 - q))) & ((1u << (2 * steps)) - 1u);
+                const uint32_t walked = (__builtin_bitreverse32(unequal) >> 1) >> (2 * (15 - q));
+                uint64_t codes = steps >= 16 ? (uint64_t)walked : (uint64_t)(walked & ((1u << (2 * steps)) - 1u));
                 r -= steps;
                 c -= steps;
                 const bool gap = stop >= 0 && r > 0 && c > 0;
-                const uint32_t choice = (choices >> (2 * (stop & 7))) & 3u;
-                codes |= gap ? (choice + 1u) << (2 * steps) : 0u;  // 'D' (code 2) or 'I' (code 3)
+                const uint32_t choice = (choices >> (2 * (stop & 15))) & 3u;
+                codes |= gap ? (uint64_t)(choice + 1u) << (2 * steps) : 0ull;  // 'D' (code 2) or 'I' (code 3)
                 ops.put_many(codes, steps + (gap ? 1 : 0));
                 const int deletion = gap && choice == 1u, insertion = gap && choice != 1u;
                 r -= deletion;
@@ -585,7 +610,7 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
                 i += insertion - deletion;
                 const bool finished = r == 0 || c == 0;
                 const bool astray = !finished && (i < 0 || i >= width);
-                if (finished || astray || (i >> 6) != word || ((c - 1) >> 3) != q8) {
+                if (finished || astray || (i >> 6) != word || ((c - 1) >> 4) != span) {
                     if (astray) {  // a proven band holds every step; this pair's did not
                         a.s.status[slot] = 3;
                         outside = true;
@@ -593,23 +618,13 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
                     live = !(finished || astray);
                     break;
                 }
-                choices = diagonal_choices(cur, i & 63);
+                if (((i & 63) >> 5) != half) read_half((i & 63) >> 5);  // the diagonal crossed bit 31/32
+                choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
             }
-            if (live) need = ((c - 1) >> 3) * words + (i >> 6);
+            if (live) need = ((c - 1) >> 4) * words + (i >> 6);
         }
-        // The chunk four rounds on: the next three rounds walk on down the diagonal's word where their chunks are
-        // the ones the lane needs by then, and wait where they are not.
-        int ahead = need;
-        ahead = n1 == ahead ? ahead - words : ahead;
-        ahead = n2 == ahead ? ahead - words : ahead;
-        ahead = n3 == ahead ? ahead - words : ahead;
-        tag = live && ahead >= 0 ? ahead : -1;
-        load_chunk(cur, tag >= 0 ? base + (size_t)tag * 128 : blank);
-        return true;
-    };
-    while (round(b0, t0, t1, t2, t3) && round(b1, t1, t2, t3, t0) && round(b2, t2, t3, t0, t1) &&
-           round(b3, t3, t0, t1, t2)) {
     }
+    take();  // the last loads: every load the wave made is waited for before it ends
     if (slot >= 0 && !outside) {
         for (; r > 0; --r) ops.put(2u);
         for (; c > 0; --c) ops.put(3u);
