@@ -569,22 +569,23 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         }
         half = h;
     };
-    // Every lane of the wave: hand the loaded pieces to their lanes through LDS, and take this lane's span.
-    const auto take = [&]() __attribute__((always_inline)) {
+    // Every lane of the wave: hand the loaded pieces to their lanes through LDS (their registers are then free for
+    // the next loads).
+    const auto hand_over = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) span_stage[wave_base + 4 * j + (lane >> 4)][lane & 15] = pieces[j];
         __builtin_amdgcn_wave_barrier();
-        read_half((i & 63) >> 5);
         now_tag = loaded_tag;
     };
     if (live) need = ((c - 1) >> 4) * words + (i >> 6);
     issue(live ? need : -1);
     while (__any(live)) {
-        take();
-        uint32_t choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
+        hand_over();
         const bool walking = live && now_tag == need;
         const int expected = walking ? need - words : need;
         issue(live && expected >= 0 ? expected : -1);
+        read_half((i & 63) >> 5);
+        uint32_t choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
         if (walking) {
             const int span = (c - 1) >> 4, word = i >> 6;
             for (;;) {
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
             if (live) need = ((c - 1) >> 4) * words + (i >> 6);
         }
     }
-    take();  // the last loads: every load the wave made is waited for before it ends
+    hand_over();  // the last loads: every load the wave made is waited for before it ends
     if (slot >= 0 && !outside) {
         for (; r > 0; --r) ops.put(2u);
         for (; c > 0; --c) ops.put(3u);
