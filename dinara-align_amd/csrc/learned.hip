@@ -28,76 +28,88 @@ namespace dinara {
 
 // ------------------------------------------------------------------ features
 
-// 4-mer counts of 64 slots a workgroup, a thread a slot, in LDS (the thread's own column, so no thread waits on
-// another). A 2-bit tape is read a word (16 letters) at a time: the 16 windows that end in the word come from the
-// word and the one before it as 64 bits, and each is counted with an LDS add whose result is not waited for. The
-// windows are counted by their tape bits (first letter lowest); a forward 4-mer code puts the first letter highest,
-// so a forward tape's counts are read back with the four 2-bit fields reversed.
+// 4-mer counts of 64 slots a workgroup, a thread a slot, in LDS (the thread's own 257-word row: the padding word
+// spreads the rows over the banks). A 2-bit tape is read a word (16 letters) at a time: the 16 windows that end in
+// the word come from the word and the one before it as 64 bits, and each is counted with an LDS add whose result
+// is not waited for. The windows are counted by their tape bits (first letter lowest); a forward 4-mer code puts
+// the first letter highest, so a forward tape's counts are read back with the four 2-bit fields reversed. The
+// feature rows are then written a slot at a time by the whole wave, 64 consecutive floats a store.
 __global__ __launch_bounds__(64) void kmer_features(Tape tape, int slots, int reversed, float *out) {
-    __shared__ int32_t hist[256 * 64];
+    constexpr int ROW = 257;
+    __shared__ int32_t hist[64 * ROW];
     const int t = threadIdx.x, slot = blockIdx.x * 64 + t;
-    for (int k = 0; k < 256; ++k) hist[k * 64 + t] = 0;
-    if (slot >= slots) return;
-    const int g = slot >> 6, col = slot & 63;
-    const uint32_t *ref_words = tape.words + tape.group_base[g] + col;
-    const uint32_t *qry_words = ref_words + (size_t)tape.group_ref_words[g] * 64;
-    const int bits = tape.bits, per = 32 / bits;
-    const int n = tape.rows[slot], m = tape.cols[slot];
-    auto count = [&](const uint32_t *words, int len, int delta) {
-        if (bits == 2) {
-            // Eight words a step, the next eight loaded while these are counted.
-            const int total = (len + 15) >> 4;
-            auto load = [&](uint32_t (&to)[8], int w0) {
+    int32_t *mine = hist + t * ROW;
+    for (int k = 0; k < 256; ++k) mine[k] = 0;
+    if (slot < slots) {
+        const int g = slot >> 6, col = slot & 63;
+        const uint32_t *ref_words = tape.words + tape.group_base[g] + col;
+        const uint32_t *qry_words = ref_words + (size_t)tape.group_ref_words[g] * 64;
+        const int bits = tape.bits, per = 32 / bits;
+        auto count = [&](const uint32_t *words, int len, int delta) {
+            if (bits == 2) {
+                // Eight words a step, the next eight loaded while these are counted.
+                const int total = (len + 15) >> 4;
+                auto load = [&](uint32_t (&to)[8], int w0) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) to[k] = w0 + k < total ? words[(size_t)(w0 + k) * 64] : 0u;
-            };
-            uint32_t now[8], next[8];
-            load(now, 0);
-            uint32_t before = 0;
-            for (int w0 = 0; w0 < total; w0 += 8) {
-                load(next, w0 + 8);
+                    for (int k = 0; k < 8; ++k) to[k] = w0 + k < total ? words[(size_t)(w0 + k) * 64] : 0u;
+                };
+                uint32_t now[8], next[8];
+                load(now, 0);
+                uint32_t before = 0;
+                for (int w0 = 0; w0 < total; w0 += 8) {
+                    load(next, w0 + 8);
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int w = w0 + k;
-                    const uint64_t pair = (uint64_t)before | ((uint64_t)now[k] << 32);
-                    // Window j starts at letter 16w + j - 3 and ends at letter 16w + j.
-                    const int first = w == 0 ? 3 : 0, last = len - 16 * w;
+                    for (int k = 0; k < 8; ++k) {
+                        const int w = w0 + k;
+                        const uint64_t pair = (uint64_t)before | ((uint64_t)now[k] << 32);
+                        // Window j starts at letter 16w + j - 3 and ends at letter 16w + j.
+                        const int first = w == 0 ? 3 : 0, last = len - 16 * w;
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        if (j >= first && j < last)
-                            atomicAdd(&hist[(uint32_t)((pair >> (26 + 2 * j)) & 255u) * 64 + t], delta);
+                        for (int j = 0; j < 16; ++j) {
+                            if (j >= first && j < last) atomicAdd(&mine[(uint32_t)((pair >> (26 + 2 * j)) & 255u)], delta);
+                        }
+                        before = now[k];
                     }
-                    before = now[k];
-                }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) now[k] = next[k];
+                    for (int k = 0; k < 8; ++k) now[k] = next[k];
+                }
+                return;
             }
-            return;
-        }
-        uint32_t window = 0;  // the last four letters, first lowest
-        int seen = 0;
-        for (int w = 0; w * per < len; ++w) {
-            uint32_t word = words[(size_t)w * 64];
-            const int here = min(per, len - w * per);
-            for (int i = 0; i < here; ++i) {
-                window = (window >> 2) | ((word & 3u) << 6);
-                word >>= bits;
-                if (++seen >= 4) atomicAdd(&hist[window * 64 + t], delta);
+            uint32_t window = 0;  // the last four letters, first lowest
+            int seen = 0;
+            for (int w = 0; w * per < len; ++w) {
+                uint32_t word = words[(size_t)w * 64];
+                const int here = min(per, len - w * per);
+                for (int i = 0; i < here; ++i) {
+                    window = (window >> 2) | ((word & 3u) << 6);
+                    word >>= bits;
+                    if (++seen >= 4) atomicAdd(&mine[window], delta);
+                }
             }
-        }
-    };
-    count(ref_words, n, 1);
-    count(qry_words, m, -1);
-    // A reversed tape holds each sequence back to front: its windows' tape bits are then the forward codes.
-    float *row = out + (size_t)slot * FEATURE_STRIDE;
-    const float scale = 8.0f / sqrtf(1.0f + (float)n + (float)m);
-    for (int k = 0; k < 256; ++k) {
-        const int at = reversed ? k : ((k & 3) << 6) | ((k & 12) << 2) | ((k & 48) >> 2) | (k >> 6);
-        row[k] = (float)hist[at * 64 + t] * scale;
+        };
+        count(ref_words, tape.rows[slot], 1);
+        count(qry_words, tape.cols[slot], -1);
     }
-    row[256] = (logf(1.0f + (float)n) - 7.0f) * 0.5f;
-    row[257] = (logf(1.0f + (float)m) - 7.0f) * 0.5f;
-    for (int k = FEATURES; k < FEATURE_STRIDE; ++k) row[k] = 0.0f;
+    __syncthreads();
+    // A reversed tape holds each sequence back to front: its windows' tape bits are then the forward codes.
+    const int first = blockIdx.x * 64, here = min(64, slots - first);
+    for (int s = 0; s < here; ++s) {
+        const int n = tape.rows[first + s], m = tape.cols[first + s];
+        const float scale = 8.0f / sqrtf(1.0f + (float)n + (float)m);
+        float *row = out + (size_t)(first + s) * FEATURE_STRIDE;
+        for (int k = t; k < FEATURE_STRIDE; k += 64) {
+            float v = 0.0f;
+            if (k < 256) {
+                const int at = reversed ? k : ((k & 3) << 6) | ((k & 12) << 2) | ((k & 48) >> 2) | (k >> 6);
+                v = (float)hist[s * ROW + at] * scale;
+            } else if (k == 256) {
+                v = (logf(1.0f + (float)n) - 7.0f) * 0.5f;
+            } else if (k == 257) {
+                v = (logf(1.0f + (float)m) - 7.0f) * 0.5f;
+            }
+            row[k] = v;
+        }
+    }
 }
 
 void launch_features(const Tape &tape, int slots, int reversed, float *out, hipStream_t stream) {
