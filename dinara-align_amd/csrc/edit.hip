@@ -488,10 +488,10 @@ __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (o
 // again for the new diagonal, until it leaves the chunk. Between rounds every lane moves to its next chunk; the two
 // chunks a diagonal run reaches next (the same word, 8 and 16 columns back) are loaded a round and two rounds
 // ahead. The moves go to the slot's tape; the CIGAR text is made from it afterwards.
-__global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
-    __shared__ u32x4 span_stage[256][16];  // each lane's span, put together by the lanes that loaded it
-    __shared__ ulonglong2 span_where[256];      // each lane's span: its two chunks' addresses
-    const int t = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots) {
+    __shared__ u32x4 span_stage[64][32];     // each lane's span, put together by the lanes that loaded it
+    __shared__ ulonglong2 span_where[64][2];  // each lane's span: its four chunks' addresses
+    const int t = blockIdx.x * 64 + threadIdx.x;
     if ((t & ~63) >= slots) return;  // whole waves only: the rounds are the wave's
     const int slot = t < slots && a.walk_ready[t] ? t : -1;
     if (slot >= 0) a.walk_ready[slot] = 0;
@@ -532,72 +532,78 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
         outside = true;
         live = false;
     }
-    // A lane's span (its next sixteen columns of one word: two 128-byte chunks) is loaded by sixteen lanes of its
-    // wave, eight to a chunk, so that each load instruction reads eight whole lines rather than a piece of 64;
-    // the pieces go to the lane through LDS. A round first takes the pieces loaded for it a round before (waiting
-    // for them), then starts the loads of the span the lane expects next (the same word, sixteen columns back) and
-    // walks while they are in flight. A lane whose next span is not the one loaded (its diagonal moved into another
-    // word) sits out a round while it comes. Span s of word w is tagged s·words + w.
-    u32x4 pieces[16];
-    uint32_t half_x[16], half_y[16];  // the span's decision planes, the 32-bit half holding the lane's diagonal
-    int half = 0;
+    // A lane's span (its next 32 columns of one word: four 128-byte chunks) is loaded by 32 lanes of its wave, eight
+    // to a chunk, so that each load instruction reads eight whole lines rather than a piece of 64; the pieces go to
+    // the lane through LDS. A round hands over the pieces loaded a round before (waiting for them), starts the loads
+    // of the span the lane expects next (the same word, 32 columns back), and walks while they are in flight, sixteen
+    // columns (a half span) at a time from LDS. A lane whose next span is not the one loaded (its diagonal moved into
+    // another word) sits out a round while it comes. Span s of word w is tagged s·words + w.
+    u32x4 pieces[32];
+    uint32_t half_x[16], half_y[16];  // sixteen columns of the span's two decision planes: the 32-bit halves that
+    int half = 0, part = 0;           // hold the lane's diagonal (`half`), in the span's `part` (0 low, 1 high)
     int now_tag = -1, loaded_tag = -1;
     int need = 0;  // the span the lane walks next
     const int chunks = (c + 7) >> 3;
-    const int lane = threadIdx.x & 63, wave_base = threadIdx.x & ~63;
+    const int lane = threadIdx.x;
     const uint64_t blank = (uint64_t)(uintptr_t)walk_blank;
-    // Every lane of the wave: publish the span's two chunk addresses, then load piece `lane % 16` of the spans of
-    // lanes 4j + lane / 16, j = 0..15.
+    // Every lane of the wave: publish the span's four chunk addresses, then load piece `lane % 32` of the spans of
+    // lanes 2j + lane / 32, j = 0..31.
     const auto issue = [&](int tag) __attribute__((always_inline)) {
-        uint64_t low = blank, high = blank;
+        uint64_t at[4] = {blank, blank, blank, blank};
         if (tag >= 0) {
             const int span = tag / words, word = tag - span * words;
-            const int lo = 2 * span, hi = 2 * span + 1 < chunks ? 2 * span + 1 : lo;  // past the pair: any chunk
-            low = (uint64_t)(uintptr_t)(base + ((size_t)lo * words + word) * 128);
-            high = (uint64_t)(uintptr_t)(base + ((size_t)hi * words + word) * 128);
-        }
-        span_where[threadIdx.x] = make_ulonglong2(low, high);
-        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const ulonglong2 where = span_where[wave_base + 4 * j + (lane >> 4)];
-            const uint64_t line = (lane & 8) ? where.y : where.x;
-            pieces[j] = *reinterpret_cast<global_chunk *>(line + (uint64_t)(lane & 7) * 16);
+            for (int k = 0; k < 4; ++k) {
+                const int chunk = 4 * span + k < chunks ? 4 * span + k : 4 * span;  // past the pair: any chunk
+                at[k] = (uint64_t)(uintptr_t)(base + ((size_t)chunk * words + word) * 128);
+            }
+        }
+        span_where[lane][0] = make_ulonglong2(at[0], at[1]);
+        span_where[lane][1] = make_ulonglong2(at[2], at[3]);
+        __builtin_amdgcn_wave_barrier();
+        const int piece = lane & 31, which = piece >> 3;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const ulonglong2 where = span_where[2 * j + (lane >> 5)][which >> 1];
+            const uint64_t line = (which & 1) ? where.y : where.x;
+            pieces[j] = *reinterpret_cast<global_chunk *>(line + (uint64_t)(piece & 7) * 16);
         }
         loaded_tag = tag;
     };
-    // This lane's span in LDS (column q: plane 0's 64 bits, then plane 1's): the half `h` of both planes.
-    const uint32_t *const mine = reinterpret_cast<const uint32_t *>(&span_stage[threadIdx.x][0]);
-    const auto read_half = [&](int h) __attribute__((always_inline)) {
+    // This lane's span in LDS (column q: plane 0's 64 bits, then plane 1's): half `h` of both planes, columns
+    // 16p .. 16p + 15.
+    const uint32_t *const mine = reinterpret_cast<const uint32_t *>(&span_stage[lane][0]);
+    const auto read_half = [&](int h, int p) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            half_x[q] = mine[4 * q + h];
-            half_y[q] = mine[4 * q + 2 + h];
+            half_x[q] = mine[4 * (16 * p + q) + h];
+            half_y[q] = mine[4 * (16 * p + q) + 2 + h];
         }
         half = h;
+        part = p;
     };
     // Every lane of the wave: hand the loaded pieces to their lanes through LDS (their registers are then free for
     // the next loads).
     const auto hand_over = [&]() __attribute__((always_inline)) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j) span_stage[wave_base + 4 * j + (lane >> 4)][lane & 15] = pieces[j];
+        for (int j = 0; j < 32; ++j) span_stage[2 * j + (lane >> 5)][lane & 31] = pieces[j];
         __builtin_amdgcn_wave_barrier();
         now_tag = loaded_tag;
     };
-    if (live) need = ((c - 1) >> 4) * words + (i >> 6);
+    if (live) need = ((c - 1) >> 5) * words + (i >> 6);
     issue(live ? need : -1);
     while (__any(live)) {
         hand_over();
         const bool walking = live && now_tag == need;
         const int expected = walking ? need - words : need;
         issue(live && expected >= 0 ? expected : -1);
-        read_half((i & 63) >> 5);
+        read_half((i & 63) >> 5, ((c - 1) >> 4) & 1);
         uint32_t choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
         if (walking) {
-            const int span = (c - 1) >> 4, word = i >> 6;
+            const int span = (c - 1) >> 5, word = i >> 6;
             for (;;) {
-                // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the span's edge,
-                // then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
+                // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the half span's
+                // edge, then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
                 const int q = (c - 1) & 15;
                 const uint32_t upto = q == 15 ? 0xFFFFFFFFu : (4u << (2 * q)) - 1u;
                 const uint32_t gaps = (choices ^ (choices >> 1)) & 0x55555555u & upto;
@@ -618,7 +624,7 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
                 i += insertion - deletion;
                 const bool finished = r == 0 || c == 0;
                 const bool astray = !finished && (i < 0 || i >= width);
-                if (finished || astray || (i >> 6) != word || ((c - 1) >> 4) != span) {
+                if (finished || astray || (i >> 6) != word || ((c - 1) >> 5) != span) {
                     if (astray) {  // a proven band holds every step; this pair's did not
                         a.s.status[slot] = 3;
                         outside = true;
@@ -626,10 +632,11 @@ __global__ __launch_bounds__(256) void edit_walk(EditArgs a, int slots) {
                     live = !(finished || astray);
                     break;
                 }
-                if (((i & 63) >> 5) != half) read_half((i & 63) >> 5);  // the diagonal crossed bit 31/32
+                // Into the span's low half, or the diagonal across bit 31/32: those columns' halves from LDS.
+                if (((i & 63) >> 5) != half || (((c - 1) >> 4) & 1) != part) read_half((i & 63) >> 5, ((c - 1) >> 4) & 1);
                 choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
             }
-            if (live) need = ((c - 1) >> 4) * words + (i >> 6);
+            if (live) need = ((c - 1) >> 5) * words + (i >> 6);
         }
     }
     hand_over();  // the last loads: every load the wave made is waited for before it ends
@@ -826,7 +833,7 @@ void launch_edit_seed(const EditArgs &a, const float *prediction, double margin,
 
 void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {
     if (slots <= 0) return;
-    hipLaunchKernelGGL(edit_walk, dim3((slots + 255) / 256), dim3(256), 0, stream, a, slots);
+    hipLaunchKernelGGL(edit_walk, dim3((slots + 63) / 64), dim3(64), 0, stream, a, slots);
     hipLaunchKernelGGL(edit_text, dim3((slots + 3) / 4), dim3(256), 0, stream, a, slots);
 }
 
