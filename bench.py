@@ -36,6 +36,11 @@ import time
 
 import numpy as np
 
+# HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default): the bins that run side by side
+# on their own streams need more to run at once. Read when the library first touches the GPU; DINARA_HW_QUEUES
+# chooses another count (4 for HIP's default).
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DINARA_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _path in (ROOT, os.path.join(ROOT, "dinara-align_amd")):
     if _path not in sys.path:
@@ -595,6 +600,21 @@ def run_uneven(args) -> int:
     n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
     m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
     print(f"[bench] uneven workload ready: {pairs} pairs in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    # Plain doubling is also timed in a child process at HIP's default 4 hardware queues (more queues let the
+    # bins run at once, which helps the other variants and can slow this one): the baseline keeps its better time.
+    child = None
+    if not args.plain_only and os.environ.get("GPU_MAX_HW_QUEUES") != "4":
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "plain.npz")
+            cmd = [sys.executable, os.path.abspath(__file__), "--workload", "uneven", "--pairs", str(pairs), "--steps",
+                   str(args.steps), "--warmup", str(args.warmup), "--plain-only", path]
+            env = dict(os.environ, DINARA_HW_QUEUES="4")
+            if subprocess.run(cmd, env=env, timeout=600).returncode == 0:
+                with np.load(path, allow_pickle=False) as z:
+                    blob, ends = z["cigars"].tobytes(), z["ends"]
+                    starts = np.concatenate(([0], ends[:-1]))
+                    child = {"ms": float(z["ms"]), "dev": float(z["dev"]), "apart": z["apart"].tolist(),
+                             "results": (z["costs"], [blob[a:b] for a, b in zip(starts, ends)])}
     handle = _lib.lib()
     if handle.dinara_device_count() < 1:
         print("[bench] no GPU visible", file=sys.stderr)
@@ -608,12 +628,33 @@ def run_uneven(args) -> int:
     handle.dinara_batch_set_keep_first(plain, 1)
     handle.dinara_batch_set_persistent(plain, 0)
     handle.dinara_batch_set_graph(plain, 0)
+    # Its bins side by side on their streams or one after another: timed, the faster kept (the stronger baseline).
+    plain_apart = {}
+    for apart in (1, 0):
+        handle.dinara_batch_set_side_by_side(plain, apart)
+        plain_apart[apart] = timed_steps(handle, plain, 3, 1)[0]
+    handle.dinara_batch_set_side_by_side(plain, min(plain_apart, key=plain_apart.get))
     plain_ms, plain_dev = timed_steps(handle, plain, args.steps, args.warmup)
     results["plain"] = edit_results(handle, plain, A, pairs)
     plain_prof = bin_profile(handle, plain)
     handle.dinara_batch_free(plain)
     del keep_plain
-    print(f"[bench] plain doubling {plain_ms:.1f} ms/step", file=sys.stderr, flush=True)
+    if args.plain_only:
+        costs, cigars = results["plain"]
+        ends = np.cumsum([len(c) for c in cigars]).astype(np.int64)
+        np.savez(args.plain_only, ms=plain_ms, dev=plain_dev, apart=np.array([plain_apart[1], plain_apart[0]]),
+                 costs=costs, cigars=np.frombuffer(b"".join(cigars), dtype=np.uint8), ends=ends)
+        print(f"[bench] plain doubling at {os.environ.get('GPU_MAX_HW_QUEUES')} hardware queues {plain_ms:.1f} "
+              f"ms/step", file=sys.stderr, flush=True)
+        return 0
+    plain_by_queues = {os.environ.get("GPU_MAX_HW_QUEUES"): round(plain_ms, 3)}
+    if child is not None:
+        results["plain_4_queues"] = child["results"]
+        plain_by_queues["4"] = round(child["ms"], 3)
+        if child["ms"] < plain_ms:
+            plain_ms, plain_dev = child["ms"], child["dev"]
+    print(f"[bench] plain doubling {plain_ms:.1f} ms/step ({plain_by_queues} by hardware queues)", file=sys.stderr,
+          flush=True)
 
     # Persistent, captured, autotuned.
     batch, keep = edit_batch(handle, refs, qrys, A, workers)
@@ -680,6 +721,27 @@ def run_uneven(args) -> int:
                     print(f"[bench] margin {margin} slope {slope}: {by_slope[(margin, slope)]:.1f} ms",
                           file=sys.stderr, flush=True)
             margin, slope = min(by_slope, key=by_slope.get)
+        # Least first bins for the two longest length classes (a long pair whose first band fails sweeps its whole
+        # length again, the run's longest chain), timed the same way.
+        by_floor = {}
+        floor = (0, 0)
+        if args.floors:
+            for item in args.floors.split(","):
+                f3, f4 = (int(v) for v in item.split(":"))
+                handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, f3, f4))
+                by_floor[(f3, f4)] = timed_steps(handle, batch, 2, 1, before=seed)[0]
+                print(f"[bench] first-bin floors {f3}:{f4}: {by_floor[(f3, f4)]:.1f} ms", file=sys.stderr, flush=True)
+            floor = min(by_floor, key=by_floor.get)
+            handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
+        # One captured graph a run, or the same launches one by one: timed on the batch, the faster kept.
+        by_launch = {}
+        for graph in (1, 0):
+            handle.dinara_batch_set_graph(batch, graph)
+            by_launch[graph] = timed_steps(handle, batch, 3, 1, before=seed)[0]
+            print(f"[bench] learned {'graph' if graph else 'launches one by one'}: {by_launch[graph]:.1f} ms",
+                  file=sys.stderr, flush=True)
+        use_graph = min(by_launch, key=by_launch.get)
+        handle.dinara_batch_set_graph(batch, use_graph)
         learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
         results["learned"] = edit_results(handle, batch, A, pairs)
         learned_proven = handle.dinara_batch_stat(batch, 4)
@@ -689,6 +751,9 @@ def run_uneven(args) -> int:
                         "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
                         "margin": margin, "slope": slope,
                         "by_margin_slope_ms": {f"{k[0]},{k[1]}": round(v, 3) for k, v in by_slope.items()},
+                        "first_bin_floors": list(floor), "graph": int(use_graph),
+                        "by_launch_ms": {("graph" if k else "one_by_one"): round(v, 3) for k, v in by_launch.items()},
+                        "by_floors_ms": {f"{k[0]}:{k[1]}": round(v, 3) for k, v in by_floor.items()},
                         "first_band_proved": round(learned_proven / pairs, 4),
                         "walked_from_first_sweep": round(walked_first / pairs, 4),
                         "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
@@ -785,6 +850,10 @@ def run_uneven(args) -> int:
                    "side_by_side": int(chosen[6]), "tune_s": round(tune_s, 1)},
         "learned": learned_info,
         "bins": {"persistent": prof, "plain_doubling": plain_prof},
+        "plain_doubling_side_by_side_ms": {("side_by_side" if k else "one_after_another"): round(v, 3)
+                                           for k, v in plain_apart.items()},
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+        "plain_doubling_ms_by_hw_queues": plain_by_queues,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -881,6 +950,10 @@ def main() -> int:
     ap.add_argument("--slope", type=float, default=0.0, help="uneven: the seeds' margin per e-fold of length")
     ap.add_argument("--slopes", default="0,0.1,0.2",
                     help="uneven: comma-separated slopes to time on the batch, keeping the fastest ('' for --slope)")
+    ap.add_argument("--plain-only", default="", help=argparse.SUPPRESS)  # the child timing plain doubling
+    ap.add_argument("--floors", default="",
+                    help="uneven: least first bins (4k-16k letters : 16k and more) timed for the learned seeds, e.g. "
+                         "'0:0,0:6,5:6' (off by default: on the uneven batch none beat 0:0)")
     ap.add_argument("--margins", default="-0.1,0,0.1",
                     help="uneven: comma-separated margins timed with each slope ('' for --margin)")
     args = ap.parse_args()
