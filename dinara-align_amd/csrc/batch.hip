@@ -209,7 +209,7 @@ struct dinara_batch {
     int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
     DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_later_lists,
         edit_later_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
-        trace_counts, trace_offset, walk_ready, walk_moves, deferred, deferred_count, deferred_held, edit_arena,
+        trace_counts, trace_offset, walk_ready, walk_moves, walk_early, deferred, deferred_count, deferred_held, edit_arena,
         arena_used, edit_stats, edit_work;
     size_t arena_capacity = 0;
     hipGraphExec_t edit_graph = nullptr;
@@ -561,7 +561,7 @@ int64_t unit_lists(dinara_batch *b) {
     DeviceBuffer *held[] = {&b->edit_first_lists, &b->edit_first_counts, &b->edit_first_low_d, &b->edit_band_low,
                             &b->edit_lists, &b->edit_counts, &b->edit_retry_lists, &b->edit_retry_counts,
                             &b->edit_later_lists, &b->edit_later_counts, &b->trace_lists, &b->trace_counts, &b->trace_offset,
-                            &b->walk_ready, &b->walk_moves, &b->deferred, &b->deferred_count, &b->deferred_held,
+                            &b->walk_ready, &b->walk_moves, &b->walk_early, &b->deferred, &b->deferred_count, &b->deferred_held,
                             &b->arena_used, &b->edit_stats, &b->edit_work};
     void *before[sizeof(held) / sizeof(held[0])];
     for (size_t i = 0; i < sizeof(held) / sizeof(held[0]); ++i) before[i] = held[i]->ptr;
@@ -585,7 +585,7 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->edit_later_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_later_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_offset.reserve(cap * 8) || !b->walk_ready.reserve(cap * 4) ||
-        !b->walk_moves.reserve(cap * 4) ||
+        !b->walk_moves.reserve(cap * 4) || !b->walk_early.reserve(cap * 4) ||
         !b->deferred.reserve(cap * 4) || !b->deferred_count.reserve(4) || !b->deferred_held.reserve(cap * 4) ||
         !b->arena_used.reserve(8) || !b->edit_stats.reserve((2 * EDIT_BINS + 2) * 4) ||
         !b->edit_work.reserve(2 * EDIT_BINS * 8))
@@ -617,6 +617,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.trace_offset = static_cast<int64_t *>(b->trace_offset.ptr);
     e.walk_ready = static_cast<int32_t *>(b->walk_ready.ptr);
     e.walk_moves = static_cast<int32_t *>(b->walk_moves.ptr);
+    e.walk_early = static_cast<int32_t *>(b->walk_early.ptr);
     e.deferred = static_cast<int32_t *>(b->deferred.ptr);
     e.deferred_count = static_cast<int32_t *>(b->deferred_count.ptr);
     e.arena = static_cast<uint8_t *>(b->edit_arena.ptr);
@@ -659,6 +660,8 @@ size_t arena_bytes(const dinara_batch *b) {
 // sweep, and for the walk.
 constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, EV_WALK = 6 * EDIT_BINS,
               EV_LATER = 6 * EDIT_BINS + 2;
+// Marks (untimed events) of the early walk's fork and join; the bins side by side use marks 0 .. 4 * EDIT_BINS + 2.
+constexpr int EARLY_WALK = 5 * EDIT_BINS;
 
 // Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
 // stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
@@ -697,6 +700,16 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace)
                  [&](int bin, hipStream_t st) {
                      launch_edit_sweep(a, bin, false, edit_grid(b, bin, b->first_count[bin]), st);
                  });
+    // Kept decisions: the pairs the first pass proved are walked on a stream of their own while the retries sweep.
+    const bool early = trace && b->keep_first;
+    if (early) {
+        launch_edit_snapshot(a, slots, ctx->stream);
+        (void)hipEventRecord(ctx->mark(EARLY_WALK), ctx->stream);
+        hipStream_t walker = ctx->side(EDIT_BINS);
+        (void)hipStreamWaitEvent(walker, ctx->mark(EARLY_WALK), 0);
+        launch_edit_walk(a, slots, walker, true);
+        (void)hipEventRecord(ctx->mark(EARLY_WALK + 1), walker);
+    }
     // The first retries side by side, filing into the later set; the later retries one bin after another.
     EditArgs retry = a;
     retry.lists = a.file_lists;
@@ -724,6 +737,7 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace)
                      [&](int bin, hipStream_t st) {
                          launch_edit_sweep(a, bin, true, edit_grid(b, bin, b->trace_reach[bin]), st);
                      });
+        if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(EARLY_WALK + 1), 0);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
         launch_edit_walk(a, slots, ctx->stream);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK + 1), ctx->stream);

@@ -58,6 +58,7 @@ struct EditArgs {
     int64_t *trace_offset;  // per slot: byte offset of its decisions in the arena
     int32_t *walk_ready;    // per slot: 1 when its decisions are kept and it waits for the walk
     int32_t *walk_moves;    // per slot: the moves the last walk made (-1: not walked by it), for its CIGAR text
+    int32_t *walk_early;    // per slot: 1 when it was ready at the first pass's end and walked while retries ran
     int32_t *deferred;      // settled pairs whose decisions found no room in the arena this pass
     int32_t *deferred_count;
     uint8_t *arena;
@@ -120,7 +121,10 @@ struct EditStart {
 // Host-side entry points of edit.hip. `grid` is the workgroups of each persistent launch (256 threads each).
 void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream);
 void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream);
-void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream);
+// `early`: walk the slots ready at the first pass's end (copied by launch_edit_snapshot), while the retries sweep on
+// other streams; it writes no flag those sweeps write. Otherwise walk every ready slot not walked early.
+void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream, bool early = false);
+void launch_edit_snapshot(const EditArgs &a, int slots, hipStream_t stream);
 // Every slot's first band and first queue from predictions of log(1 + cost) in device memory (learned seeds).
 void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, double slope, const EditStart &start,
                       int slots, hipStream_t stream);

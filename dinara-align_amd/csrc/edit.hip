@@ -488,13 +488,22 @@ __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (o
 // again for the new diagonal, until it leaves the chunk. Between rounds every lane moves to its next chunk; the two
 // chunks a diagonal run reaches next (the same word, 8 and 16 columns back) are loaded a round and two rounds
 // ahead. The moves go to the slot's tape; the CIGAR text is made from it afterwards.
-__global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots) {
+__global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early) {
     __shared__ u32x4 span_stage[64][32];     // each lane's span, put together by the lanes that loaded it
     __shared__ ulonglong2 span_where[64][2];  // each lane's span: its four chunks' addresses
     const int t = blockIdx.x * 64 + threadIdx.x;
     if ((t & ~63) >= slots) return;  // whole waves only: the rounds are the wave's
-    const int slot = t < slots && a.walk_ready[t] ? t : -1;
-    if (slot >= 0) a.walk_ready[slot] = 0;
+    int slot = -1;
+    if (t < slots) {
+        if (early) {
+            slot = a.walk_early[t] ? t : -1;  // read only: the retries' sweeps write `walk_ready` meanwhile
+        } else if (a.walk_early[t]) {
+            a.walk_ready[t] = 0;  // walked early
+        } else if (a.walk_ready[t]) {
+            slot = t;
+            a.walk_ready[t] = 0;
+        }
+    }
     // A wave walking a long pair issues first: its rounds, one after another, are the walk's longest chain.
     {
         int letters = slot >= 0 ? a.s.tape.rows[slot] + a.s.tape.cols[slot] : 0;
@@ -684,6 +693,7 @@ __global__ void edit_reset(EditArgs a, int slots) {
         a.s.status[t] = -1;
         a.band_low[t] = a.first_band_low[t];
         a.walk_ready[t] = 0;
+        a.walk_early[t] = 0;
     }
     for (int bin = 0; bin < EDIT_BINS; ++bin)
         if (t < a.first_counts[bin])
@@ -705,6 +715,12 @@ __global__ void edit_reset(EditArgs a, int slots) {
         *a.arena_used = 0;
         if (a.s.cigar_used) *a.s.cigar_used = 0;
     }
+}
+
+// The slots ready at the first pass's end, for the early walk.
+__global__ void edit_snapshot(EditArgs a, int slots) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < slots) a.walk_early[t] = a.walk_ready[t];
 }
 
 // A further tracing pass: the arena and the trace and walk queues emptied (one workgroup, before the requeue).
@@ -809,6 +825,10 @@ void sweep_bin(const EditArgs &a, int bin, int grid, hipStream_t stream) {
 
 }  // namespace
 
+void launch_edit_snapshot(const EditArgs &a, int slots, hipStream_t stream) {
+    if (slots > 0) hipLaunchKernelGGL(edit_snapshot, dim3((slots + 255) / 256), dim3(256), 0, stream, a, slots);
+}
+
 void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream) {
     const int threads = slots > 2 * EDIT_BINS + 2 ? slots : 2 * EDIT_BINS + 2;
     hipLaunchKernelGGL(edit_reset, dim3((threads + 255) / 256), dim3(256), 0, stream, a, slots);
@@ -831,9 +851,9 @@ void launch_edit_seed(const EditArgs &a, const float *prediction, double margin,
                            start, slots);
 }
 
-void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream) {
+void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream, bool early) {
     if (slots <= 0) return;
-    hipLaunchKernelGGL(edit_walk, dim3((slots + 63) / 64), dim3(64), 0, stream, a, slots);
+    hipLaunchKernelGGL(edit_walk, dim3((slots + 63) / 64), dim3(64), 0, stream, a, slots, early ? 1 : 0);
     hipLaunchKernelGGL(edit_text, dim3((slots + 3) / 4), dim3(256), 0, stream, a, slots);
 }
 
