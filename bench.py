@@ -789,6 +789,16 @@ def run_uneven(args) -> int:
                 "note": "proven-band cells at the true cost (strict) x 56 int32 ops a 64-cell word (the bit-parallel "
                         "step and its 2-bit decisions); the sweeps' elapsed time in a run launched kernel by kernel, "
                         "less the walk"}
+    # The same sweeps against HBM: the 2-bit decisions the walk needs (proven-band cells, algorithmic), and those
+    # the sweeps write (every word-column swept keeping decisions, 16 bytes each).
+    bins_now = learned_info["bins"] if learned_info is not None else prof
+    swept = float(sum(bins_now["sweep_word_columns"]) + sum(bins_now["trace_word_columns"])) * 16
+    needed = float(proven.sum()) / 4
+    roofline["hbm_writes"] = {"bound": "hbm", "achieved": round(needed / (sweeps_ms / 1e3) / 1e12, 3),
+                              "written": round(swept / (sweeps_ms / 1e3) / 1e12, 3), "peak": 8.0, "unit": "TB/s",
+                              "frac": round(needed / (sweeps_ms / 1e3) / 8e12, 4),
+                              "note": "decision bytes: proven-band cells x 2 bits (achieved) and word-columns swept "
+                                      "keeping decisions x 16 B (written), over the same time"}
 
     cpu = None
     exact_cpu = None
