@@ -216,6 +216,8 @@ struct dinara_batch {
     double edit_ms[EDIT_BINS] = {}, trace_ms[EDIT_BINS] = {}, edit_sweeps[EDIT_BINS] = {},
            trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0, walked_first = 0;
     double sweep_work[EDIT_BINS] = {}, trace_work[EDIT_BINS] = {};  // word-columns swept by each bin
+    bool stats_pending = false;  // the unit path's device counters of the last run, read when first asked for
+    int64_t stats_slots = 0;
 };
 
 namespace {
@@ -820,15 +822,11 @@ int64_t run_unit(dinara_batch *b, bool trace) {
     (void)hipEventElapsedTime(&sweeps, ctx->start, ctx->sweep_stop);
     b->run_ms = total;
     b->sweep_ms = sweeps;
-    uint32_t stats[2 * EDIT_BINS + 2] = {};
-    unsigned long long work[2 * EDIT_BINS] = {};
-    if (!b->edit_stats.read(ctx, stats, sizeof(stats)) || !b->edit_work.read(ctx, work, sizeof(work)))
-        return DINARA_OUT_OF_MEMORY;
+    // The device counters (sweeps, word-columns, first bands proven) are read back only when asked for: a
+    // short batch's run is then one graph launch and one wait.
+    b->stats_pending = true;
+    b->stats_slots = slots;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
-        b->edit_sweeps[bin] = stats[bin];
-        b->trace_sweeps[bin] = stats[EDIT_BINS + bin];
-        b->sweep_work[bin] = (double)work[bin];
-        b->trace_work[bin] = (double)work[EDIT_BINS + bin];
         if (!graph) {
             float ms = 0;
             if (b->first_count[bin] > 0 &&
@@ -850,8 +848,6 @@ int64_t run_unit(dinara_batch *b, bool trace) {
         (void)hipEventElapsedTime(&ms, ctx->event(EV_WALK), ctx->event(EV_WALK + 1));
         b->walk_ms = ms;
     }
-    b->first_proven = (double)(slots - stats[2 * EDIT_BINS]);
-    b->walked_first = stats[2 * EDIT_BINS + 1];
     b->launches = graph ? 1 : 1 + EDIT_BINS * (trace ? 2 : 1) + (trace ? 1 : 0);
     b->ran = true;
     return 0;
@@ -1487,6 +1483,26 @@ void dinara_batch_free(dinara_batch *batch) {
 
 double dinara_batch_stat(dinara_batch *b, int64_t what) {
     if (!b) return 0;
+    const bool counted = what == 4 || what == 9 ||
+                         (what >= 700 && what < 1200 && what / 100 != 8 && what % 100 < EDIT_BINS);
+    if (counted && b->stats_pending) {  // the unit path's counters of the last run
+        DeviceContext *ctx = b->ctx;
+        std::lock_guard<std::mutex> serial(ctx->lock);
+        DeviceGuard guard(ctx);
+        uint32_t stats[2 * EDIT_BINS + 2] = {};
+        unsigned long long work[2 * EDIT_BINS] = {};
+        if (b->edit_stats.read(ctx, stats, sizeof(stats)) && b->edit_work.read(ctx, work, sizeof(work))) {
+            for (int bin = 0; bin < EDIT_BINS; ++bin) {
+                b->edit_sweeps[bin] = stats[bin];
+                b->trace_sweeps[bin] = stats[EDIT_BINS + bin];
+                b->sweep_work[bin] = (double)work[bin];
+                b->trace_work[bin] = (double)work[EDIT_BINS + bin];
+            }
+            b->first_proven = (double)(b->stats_slots - stats[2 * EDIT_BINS]);
+            b->walked_first = stats[2 * EDIT_BINS + 1];
+        }
+        b->stats_pending = false;
+    }
     switch (what) {
         case 0: return b->run_ms;
         case 1: return b->sweep_ms;

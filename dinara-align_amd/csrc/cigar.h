@@ -58,6 +58,73 @@ __device__ __forceinline__ unsigned long long wave_claim(unsigned long long *cou
     return base + inclusive - length;
 }
 
+// The CIGAR text of one lane's tape, by that lane alone (the form for short tapes, where a whole wave on each
+// pair costs more than it saves): runs of one move, each its length then its letter, in the alignment's order (the
+// moves' own order when the sequences were walked reversed, else back to front). Returns the text's length;
+// writes it to `out` unless null. Runs are found a word at a time: the moves that differ from the run's are the
+// nonzero 2-bit fields of the word xor the run's code repeated.
+__device__ __forceinline__ int64_t lane_cigar_text(int eqx, int forward, const uint64_t *words, int64_t count,
+                                                   char *out) {
+    const uint32_t letters = (uint32_t)(eqx ? '=' : 'M') | ((uint32_t)'X' << 8) | ((uint32_t)'D' << 16) |
+                             ((uint32_t)'I' << 24);
+    int64_t length = 0;
+    auto emit = [&](uint32_t code, int64_t run) {
+        const uint32_t v = (uint32_t)run;
+        const int digits = decimal_digits(v);
+        if (out) {
+            uint32_t rest = v;
+            for (int i = digits - 1; i >= 0; --i) {
+                out[length + i] = (char)('0' + rest % 10u);
+                rest /= 10u;
+            }
+            out[length + digits] = (char)(letters >> (8 * code));
+        }
+        length += digits + 1;
+    };
+    auto code_at = [&](int64_t p) { return (uint32_t)(words[p >> 5] >> (2 * (p & 31))) & 3u; };
+    if (forward) {
+        int64_t p = 0;
+        while (p < count) {
+            const uint32_t code = code_at(p);
+            const uint64_t repeated = (uint64_t)code * 0x5555555555555555ull;
+            int64_t q = p;
+            while (q < count) {
+                const int off = (int)(q & 31);
+                const uint64_t x = (words[q >> 5] ^ repeated) >> (2 * off);
+                if (x == 0) {
+                    q += 32 - off;
+                    continue;
+                }
+                q += __builtin_ctzll(x) >> 1;
+                break;
+            }
+            q = q < count ? q : count;
+            emit(code, q - p);
+            p = q;
+        }
+    } else {
+        int64_t p = count - 1;
+        while (p >= 0) {
+            const uint32_t code = code_at(p);
+            const uint64_t repeated = (uint64_t)code * 0x5555555555555555ull;
+            int64_t q = p;
+            while (q >= 0) {
+                const int off = (int)(q & 31);
+                const uint64_t x = (words[q >> 5] ^ repeated) << (2 * (31 - off));
+                if (x == 0) {
+                    q -= off + 1;
+                    continue;
+                }
+                q -= __builtin_clzll(x) >> 1;
+                break;
+            }
+            emit(code, p - q);
+            p = q;
+        }
+    }
+    return length;
+}
+
 // Moves at alignment positions p0 .. p0 + 31 of a tape of `count` moves (p0 a multiple of 32 below count), field k
 // (bits 2k, 2k + 1) the move at p0 + k, zero past the end. The alignment takes the tape's order when `forward` (the
 // sequences were walked reversed), else its reverse: position p is then tape move count - 1 - p, so the 32 moves
@@ -183,13 +250,30 @@ __device__ int64_t wave_cigar_text(int eqx, int forward, const uint64_t *words, 
 }
 
 // Writes the CIGAR of each lane's tape (`moves` moves at `words`; `moves` below zero for a lane with none) into the
-// shared output: the texts' lengths first, one tape at a time with the whole wave, then one atomic claims the
-// wave's room (an exclusive prefix sum across the lanes) and the texts are written the same way. Every lane of the
-// wave must call it.
+// shared output: the texts' lengths first, then one atomic claims the wave's room (an exclusive prefix sum across
+// the lanes) and the texts are written. Long tapes are made one at a time by the whole wave, short ones by their
+// own lanes. Every lane of the wave must call it.
 __device__ __forceinline__ void publish_wave(int eqx, int forward, const uint64_t *words, int64_t moves,
                                              char *cigars, unsigned long long *used, int64_t *offset,
                                              int32_t *length) {
     const int lane = threadIdx.x & 63;
+    // Short tapes (none past one 2048-move step): each lane makes its own text.
+    int64_t longest = moves;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t other = __shfl_xor(longest, d);
+        longest = other > longest ? other : longest;
+    }
+    if (longest <= 2048) {
+        const int64_t text = moves >= 0 ? lane_cigar_text(eqx, forward, words, moves, nullptr) : 0;
+        const unsigned long long at = wave_claim(used, (unsigned long long)text);
+        if (moves >= 0) {
+            lane_cigar_text(eqx, forward, words, moves, cigars + at);
+            *offset = (int64_t)at;
+            *length = (int32_t)text;
+        }
+        return;
+    }
     const unsigned long long walked = __ballot(moves >= 0);
     const unsigned long long tape = (unsigned long long)(uintptr_t)words;
     int64_t mine = 0;

@@ -175,6 +175,15 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     int32_t *counts = TRACE ? a.trace_counts : a.counts;
     const int count = counts[bin];
     int32_t *head = counts + EDIT_BINS + bin;
+    // Narrow bins hold many short pairs a wave: a fixed priority by width. Waves of 16 lanes a pair and more set
+    // theirs from the longest pair they have left, at each claim (below).
+    if constexpr (L < 16) {
+        if (W >= 512)
+            __builtin_amdgcn_s_setprio(2);
+        else if (W >= 128)
+            __builtin_amdgcn_s_setprio(1);
+    }
+
     int slot = -1;
     bool drained = false;
     int n = 0, m = 0, lo = 0, hi = 0, j = 0, top = 0;
@@ -267,20 +276,22 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             }
             // The wave's longest pair left, column after column, is the run's longest chain when it is long: such
             // waves issue first where they share a SIMD (set again at each claim, as pairs end and new ones start).
-            int left = slot >= 0 ? m - j : 0;
+            if constexpr (L >= 16) {
+                int left = slot >= 0 ? m - j : 0;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int other = __shfl_xor(left, d);
-                left = other > left ? other : left;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int other = __shfl_xor(left, d);
+                    left = other > left ? other : left;
+                }
+                if (left > 12000)
+                    __builtin_amdgcn_s_setprio(3);
+                else if (left > 4000)
+                    __builtin_amdgcn_s_setprio(2);
+                else if (left > 1000)
+                    __builtin_amdgcn_s_setprio(1);
+                else
+                    __builtin_amdgcn_s_setprio(0);
             }
-            if (left > 12000)
-                __builtin_amdgcn_s_setprio(3);
-            else if (left > 4000)
-                __builtin_amdgcn_s_setprio(2);
-            else if (left > 1000)
-                __builtin_amdgcn_s_setprio(1);
-            else
-                __builtin_amdgcn_s_setprio(0);
         }
         if (!__ballot(slot >= 0)) break;
 
