@@ -491,14 +491,14 @@ typedef const u32x4 __attribute__((address_space(1))) global_chunk;
 __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (one address space, global loads)
 
 // One thread a slot, walking it if it is ready: from (n, m) back to the origin over the 2-bit decisions. Slots are
-// in length order, longest first, so a wave's pairs walk about as far as each other.
+// in length order, longest first, so a wave's pairs walk about as far as each other. `early`: the slots ready at
+// the first pass's end (see launch_edit_walk); otherwise every ready slot not walked early.
 //
-// The walk goes by rounds, the wave's lanes together. In a round a lane stays in one chunk (eight columns of one
-// word, 128 bytes in registers): it takes the 2-bit choices of its diagonal in the chunk's eight columns at once,
-// makes every diagonal step up to the next gap or the chunk's edge in one go, makes the gap, and takes the choices
-// again for the new diagonal, until it leaves the chunk. Between rounds every lane moves to its next chunk; the two
-// chunks a diagonal run reaches next (the same word, 8 and 16 columns back) are loaded a round and two rounds
-// ahead. The moves go to the slot's tape; the CIGAR text is made from it afterwards.
+// The walk goes by rounds, the wave's lanes together. In a round a lane stays in one span (32 columns of one
+// word), sixteen columns at a time: it takes the 2-bit choices of its diagonal in those sixteen columns at once,
+// makes every diagonal step up to the next gap or the half span's edge in one go, makes the gap, and takes the
+// choices again for the new diagonal, until it leaves the span. The moves go to the slot's tape; the CIGAR text is
+// made from it afterwards (edit_text).
 __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early) {
     __shared__ u32x4 span_stage[64][32];     // each lane's span, put together by the lanes that loaded it
     __shared__ ulonglong2 span_where[64][2];  // each lane's span: its four chunks' addresses
