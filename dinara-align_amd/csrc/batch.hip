@@ -965,11 +965,8 @@ int64_t fetch(dinara_batch *b) {
     return 0;
 }
 
-// A pair's code when the device left it: under a cap, a band no alignment fits leaves nothing within it.
-int64_t unsettled_code(const dinara_batch *b, int32_t status) {
-    if (status == 2) return DINARA_OUT_OF_MEMORY;
-    return DINARA_OUT_OF_MEMORY;
-}
+// A pair's code when the device left it unsettled (a nonzero status): no result.
+int64_t unsettled_code(int32_t) { return DINARA_OUT_OF_MEMORY; }
 
 dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *const *refs, const int64_t *ref_lens,
                         const char *const *qrys, const int64_t *qry_lens, const dinara_costs *costs,
@@ -1088,7 +1085,7 @@ int64_t dinara_batch_distances(dinara_batch *b, int64_t *results) {
     for (size_t s = 0; s < b->slot_pair.size(); ++s) {
         const int32_t p = b->slot_pair[s];
         if (b->status_h[s] != 0) {
-            results[p] = unsettled_code(b, b->status_h[s]);
+            results[p] = unsettled_code(b->status_h[s]);
             continue;
         }
         const int64_t cost = b->cost_h[s];
@@ -1116,7 +1113,7 @@ int64_t dinara_batch_alignments(dinara_batch *b, dinara_alignment *alignments, i
             out.cigar = nullptr;
             out.cigar_length = 0;
             if (b->status_h[s] != 0) {
-                statuses[p] = unsettled_code(b, b->status_h[s]);
+                statuses[p] = unsettled_code(b->status_h[s]);
                 continue;
             }
             const int64_t cost = b->cost_h[s];
