@@ -3,6 +3,7 @@ restatement (oracle/mlp.py), training on costs the aligner itself computes, and 
 network's seeds the band certificate proves more pairs in their first sweep, and every distance is unchanged."""
 
 import ctypes
+import random
 
 import numpy as np
 import pytest
@@ -186,3 +187,20 @@ def test_doubling_policy_changes_no_result():
             h.dinara_free(found[i].cigar)
         h.dinara_batch_free(b)
     assert results[0] == results[1]
+
+
+def test_features_of_long_sequences_match_numpy():
+    # A sequence past 32,767 letters takes the 32-bit counters (shorter batches pack two 16-bit counts a word): a
+    # 4-mer counted more than 32,767 times, and ordinary long pairs, in both tape orders.
+    rng = random.Random(61)
+    long_ref = "".join(rng.choice("ACGT") for _ in range(34000))
+    refs = [s.encode() for s in ("A" * 33000, long_ref, "ACGTTGCA" * 40)]
+    qrys = [s.encode() for s in ("A" * 10, long_ref[:20000] + "T" * 50 + long_ref[20000:], "ACGTTGCA" * 39)]
+    for kind in (0, 1):
+        b = Batch(refs, qrys, kind=kind)
+        try:
+            got = b.features()
+        finally:
+            b.free()
+        for i, (r, q) in enumerate(zip(refs, qrys)):
+            assert np.allclose(got[i], mlp.features(r, q), rtol=1e-5, atol=1e-6), (kind, i)
