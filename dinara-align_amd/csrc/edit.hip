@@ -515,8 +515,9 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
             a.walk_ready[t] = 0;
         }
     }
-    // A wave walking a long pair issues first: its rounds, one after another, are the walk's longest chain.
-    {
+    // A wave walking a long pair issues first: its rounds, one after another, are the walk's longest chain. Not in
+    // the early walk, which has until the retries end, whose sweeps are then the longest chains.
+    if (!early) {
         int letters = slot >= 0 ? a.s.tape.rows[slot] + a.s.tape.cols[slot] : 0;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
