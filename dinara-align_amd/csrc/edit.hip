@@ -174,6 +174,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     const int32_t *queue = (TRACE ? a.trace_lists : a.lists) + (size_t)bin * a.capacity;
     int32_t *counts = TRACE ? a.trace_counts : a.counts;
     const int count = counts[bin];
+    if (count <= 0) return;  // an empty queue (the tracing pass, when every pair kept its decisions)
     int32_t *head = counts + EDIT_BINS + bin;
     // Narrow bins hold many short pairs a wave: a fixed priority by width. Waves of 16 lanes a pair and more set
     // theirs from the longest pair they have left, at each claim (below).
@@ -218,7 +219,11 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
         if (askers) {
             const int leader = __builtin_ctzll(askers);
             int base = 0;
-            if (lane == leader) base = atomicAdd(head, __popcll(askers));
+            // A drained queue is seen by a plain read (the head only grows): no atomic for it.
+            if (lane == leader) {
+                base = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (base < count) base = atomicAdd(head, __popcll(askers));
+            }
             base = __shfl(base, leader);
             const int mine = base + __popcll(askers & ((1ull << lane) - 1ull));
             const int idx = __shfl(mine, lane - k);
