@@ -1407,7 +1407,10 @@ int64_t dinara_batch_seed_learned_by_length(dinara_batch *b, dinara_mlp *net, do
     return seed_slots(b, [&](int64_t s) {
         const double longer = std::max(1000.0, (double)std::max(b->rows[s], b->cols[s]));
         const double guess = std::expm1((double)predicted[s] + margin + slope * std::log(longer / 1000.0));
-        return guess > 0 ? (int64_t)guess : int64_t(0);
+        // Clamped before the cast: past 2^40 every band is the whole matrix anyway, and an infinite or huge
+        // prediction must not reach a conversion it does not fit (NaN fails the test and gives 0).
+        constexpr double most = 1099511627776.0;
+        return guess > 0 ? (int64_t)(guess < most ? guess : most) : int64_t(0);
     });
 }
 

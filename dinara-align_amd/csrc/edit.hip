@@ -784,7 +784,11 @@ __global__ void edit_seed(EditArgs a, const float *prediction, double margin, do
         const double longer_side = n > m ? n : m;
         const double guessed =
             expm1((double)prediction[t] + margin + slope * log((longer_side > 1000 ? longer_side : 1000) / 1000.0));
-        const long long g = guessed > 0 ? (long long)guessed : 0;
+        // Clamped before the cast: a guess above n + m + 2 proves nothing more, and an infinite or huge prediction
+        // from a badly trained network must not reach a conversion to an integer it does not fit (NaN fails both
+        // tests and gives 0).
+        const double most = (double)n + (double)m + 2.0;
+        const long long g = guessed > 0 ? (long long)(guessed < most ? guessed : most) : 0;
         const long long from_up = end > 0 ? end : 0, from_down = end < 0 ? end : 0;
         long long up = (g + end) >= 0 ? (g + end) / 2 : -((-(g + end) + 1) / 2);  // floor((g + end) / 2)
         long long down = (end - g) >= 0 ? (end - g + 1) / 2 : -((g - end) / 2);   // ceil((end - g) / 2)

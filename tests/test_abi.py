@@ -50,6 +50,11 @@ def test_alignment_helpers():
     assert found.gapped("GGGGACGTACGTGGGG", "CCCCACGTTCGTCC") == ("ACGTACGT", "ACGTTCGT")
     assert found.clipped_cigar(14) == "4S4=1X3=2S"
     assert found.edit_distance("GGGGACGTACGTGGGG", "CCCCACGTTCGTCC") == 1
+    # The reference's own literal (tests/test_python.py:90).
+    assert found.mismatch_string("GGGGACGTACGTGGGG", "CCCCACGTTCGTCC") == "4A3"
+    assert found.identity("GGGGACGTACGTGGGG", "CCCCACGTTCGTCC") == 7 / 8
+    assert found.runs() == [(4, "="), (1, "X"), (3, "=")]
+    assert found.clipped_cigar(14, hard=True) == "4H4=1X3=2H"
     assert da.Band.around(3) == da.Band(-3, 3)
     assert da.Costs.affine(4, 6, 2) == da.Costs(4, 6, 2)
     with pytest.raises(ValueError):
@@ -57,9 +62,28 @@ def test_alignment_helpers():
     with pytest.raises(ValueError):
         da.Costs.affine(4, 6, 2).with_deletions(6, 0)
     with pytest.raises(TypeError):
-        da._text(5)
+        da.types.as_bytes(5)
     with pytest.raises(ValueError):
-        da._text("ACGTé")
+        da.types.as_bytes("ACGTé")
+
+
+def test_sam_mismatch_string_follows_the_sam_grammar():
+    """MD by the SAM specification: [0-9]+(([A-Z]|\\^[A-Z]+)[0-9]+)*, insertions left out, a deleted run's
+    letters after one `^`, a count (possibly 0) between every two tokens."""
+    import dinara_align as da
+    ref, qry = "ACGTTTGCAGGA", "ACTTGCCAGTA"
+    found = da.Alignment(0, 0, "2=2D4=1I2=1X1=", 0, 12, 0, 11)
+    assert found.gapped(ref, qry) == ("ACGTTTGC-AGGA", "AC--TTGCCAGTA")
+    assert found.mismatch_string(ref, qry) == "2^GT6G1"
+    assert found.edit_distance(ref, qry) == 4
+    # A deletion straight after a substitution, and a substitution straight after a deletion.
+    tail = da.Alignment(0, 0, "1X1D1X", 0, 3, 0, 2)
+    assert tail.mismatch_string("ACG", "TT") == "0A0^C0G0"
+    # An insertion between two deletions closes the first run.
+    split = da.Alignment(0, 0, "1D1I1D", 0, 2, 0, 1)
+    assert split.mismatch_string("AC", "G") == "0^A0^C0"
+    assert da.Alignment(0, 0, "", 0, 0, 0, 0).mismatch_string("", "") == "0"
+    assert da.Alignment(0, 0, "", 0, 0, 0, 0).identity("", "") == 0.0
 
 
 def test_synthetic_generators_are_seeded():

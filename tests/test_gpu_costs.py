@@ -185,7 +185,7 @@ def class_pairs(refs, qrys, costs, kind):
     from dinara_align import _lib
     handle = _lib.lib()
     R, Q = _lib.Texts(refs), _lib.Texts(qrys)
-    c = costs._c()
+    c = costs.as_c()
     status = ctypes.c_int64(0)
     batch = handle.dinara_batch_new(0, kind, len(refs), *R.args(), *Q.args(), ctypes.byref(c), None, None, 4,
                                     ctypes.byref(status))

@@ -190,8 +190,8 @@ def test_doubling_policy_changes_no_result():
 
 
 def test_features_of_long_sequences_match_numpy():
-    # A sequence past 32,767 letters takes the 32-bit counters (shorter batches pack two 16-bit counts a word): a
-    # 4-mer counted more than 32,767 times, and ordinary long pairs, in both tape orders.
+    # Long sequences: a 4-mer counted more than 32,767 times (past any 16-bit count), and ordinary long pairs, in
+    # both tape orders.
     rng = random.Random(61)
     long_ref = "".join(rng.choice("ACGT") for _ in range(34000))
     refs = [s.encode() for s in ("A" * 33000, long_ref, "ACGTTGCA" * 40)]
@@ -204,3 +204,38 @@ def test_features_of_long_sequences_match_numpy():
             b.free()
         for i, (r, q) in enumerate(zip(refs, qrys)):
             assert np.allclose(got[i], mlp.features(r, q), rtol=1e-5, atol=1e-6), (kind, i)
+
+
+@pytest.mark.parametrize("unit", [False, True])
+def test_unbounded_predictions_seed_whole_bands_and_change_nothing(unit):
+    # A network whose output is +inf (its last bias), NaN or huge: the guessed cost is clamped before it becomes an
+    # integer, the first band is then the widest, and every distance stays the oracle's.
+    from oracle import oracle as o
+    refs, qrys = synthetic.uneven(200, seed=24, clip=(50, 1200))
+    h = _lib.lib()
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    costs = _lib.CCosts(1, 0, 1, -1, 0, 0, 0, -1, 0) if unit else _lib.CCosts(4, 6, 2, -1, 0, 0, 0, -1, 0)
+    want_costs = o.Costs.affine(1, 0, 1) if unit else o.Costs.affine(4, 6, 2)
+    want = [o.costs_cost(r, q, want_costs) for r, q in zip(refs, qrys)]
+    net = Net(3)
+    try:
+        for bias in (np.inf, np.nan, 1e30, -np.inf):
+            w = net.weights()
+            w[-257:-1] = 0.0  # the output layer's weights: the prediction is its bias alone
+            w[-1] = bias
+            assert h.dinara_mlp_set_weights(net.ptr, w.ctypes.data) == 0
+            status = ctypes.c_int64(0)
+            b = h.dinara_batch_new(0, 0, len(refs), *R.args(), *Q.args(), ctypes.byref(costs), None, None, 4,
+                                   ctypes.byref(status))
+            assert b, status.value
+            try:
+                assert h.dinara_batch_seed_learned_by_length(b, net.ptr, ctypes.c_double(0.1),
+                                                             ctypes.c_double(0.1)) == 0
+                assert h.dinara_batch_run(b) == 0
+                out = (ctypes.c_int64 * len(refs))()
+                assert h.dinara_batch_distances(b, out) == 0
+                assert list(out) == want, bias
+            finally:
+                h.dinara_batch_free(b)
+    finally:
+        net.free()
