@@ -83,6 +83,10 @@ def test_sam_mismatch_string_follows_the_sam_grammar():
     split = da.Alignment(0, 0, "1D1I1D", 0, 2, 0, 1)
     assert split.mismatch_string("AC", "G") == "0^A0^C0"
     assert da.Alignment(0, 0, "", 0, 0, 0, 0).mismatch_string("", "") == "0"
+    # The reference's literals (tests/test_alignment.mojo:2794-2797).
+    gapped = da.Alignment(3, -3, "3M2D4M1I2M", 0, 11, 0, 10)
+    assert gapped.clipped_cigar(10) == "3M2D4M1I2M"
+    assert gapped.mismatch_string("ACGTTTGCAAC", "ACGTGCATAC") == "3^TT6"
     assert da.Alignment(0, 0, "", 0, 0, 0, 0).identity("", "") == 0.0
 
 

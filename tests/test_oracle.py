@@ -50,6 +50,10 @@ DNA = o.uniform_scoring(2, -4)
         # tests/test_c_api.cpp:127-128 (two-piece)
         ("GATTACAGCTTGCA" + "C" * 30 + "TGGACCATGAGTCATTGACCAGTCGATC",
          "GATTACAGCTTGCATGGACCATGAGTCAGTTGACCAGTCGATC", TWO, False, True, (62, "14=30D14=1I14=")),
+        # tests/test_alignment.mojo:1185 (letters outside ACGT match themselves)
+        ("acgu", "acgu", AFFINE, False, True, (0, "4=")),
+        # tests/test_alignment.mojo:2794-2795
+        ("ACGTTTGCAAC", "ACGTGCATAC", EDIT, False, False, (3, "3M2D4M1I2M")),
         # tests/test_c_api.cpp:139 (deletions dearer than insertions)
         ("ACGACGT", "ACGTACGT", o.Costs.affine(4, 2, 1).with_deletions(20, 5), False, True, (3, "3=1I4=")),
     ],
