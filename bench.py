@@ -600,7 +600,7 @@ def run_uneven(args) -> int:
     n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
     m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
     print(f"[bench] uneven workload ready: {pairs} pairs in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    # The training pairs are made here too, before this process first touches the GPU: their generator forks.
+    # The training pairs are made here too, before the first GPU call, so the run's memory peak comes first.
     train_r = train_q = None
     if train_n > 0 and not args.plain_only:
         t0 = time.time()

@@ -108,9 +108,9 @@ def uneven_parallel(pairs: int, seed: int = 3, clip=(100, 20_000), chunk: int = 
         parts = [_uneven_chunk(j) for j in jobs]
     else:
         import multiprocessing
-        # Call before the process first touches the GPU: the workers are forked. The pool is closed and joined,
+        # Spawned, not forked: safe after the calling process has touched the GPU. The pool is closed and joined,
         # not terminated, so each worker exits on its own.
-        pool = multiprocessing.get_context("fork").Pool(min(workers, len(jobs)))
+        pool = multiprocessing.get_context("spawn").Pool(min(workers, len(jobs)))
         try:
             parts = pool.map(_uneven_chunk, jobs)
         finally:

@@ -615,3 +615,251 @@ int64_t orc_scoring_local_align(const uint8_t *ref, int64_t n, const uint8_t *qr
     spans[3] = qs + qm;
     return score;
 }
+
+/* ---------------------------------------------------------------- gap costs over a proven band (long pairs) */
+
+/* The least any path through library diagonal k costs (lanes.mojo:333-345 in library terms, k = reference
+ * letters less query letters, the end on n - m). */
+static int64_t beyond(const orc_costs *costs, int64_t n, int64_t m, int64_t k) {
+    return orc_off_band(costs, -k, m - n);
+}
+
+/* costs_sweep's recurrence and flag priority (lanes.mojo:795-947, 905-923) over library diagonals lo..hi alone:
+ * values in rolling rows, flags (when asked) in (n+1) rows of hi-lo+1 bytes, row r's cell c at r*width + c-r+hi.
+ * The band must hold diagonals 0 and n-m. Returns the corner's cost. */
+static int64_t band_sweep(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                          int64_t lo, int64_t hi, uint8_t *flags) {
+    const int64_t width = hi - lo + 1;
+    const size_t slots = (size_t)width + 2;
+    int64_t *mem = malloc(sizeof(int64_t) * slots * 8);
+    int64_t *ph = mem, *pd = mem + slots, *pd2 = mem + 2 * slots, *ch = mem + 3 * slots, *cd = mem + 4 * slots,
+            *cd2 = mem + 5 * slots, *ci = mem + 6 * slots, *ci2 = mem + 7 * slots;
+    for (size_t i = 0; i < slots * 8; i++) mem[i] = INF;
+    int64_t corner = INF;
+    for (int64_t r = 0; r <= n; r++) {
+        /* Slot j+1 holds cell c = r - hi + j; the row above's cell c sits one slot right, its c-1 in slot j+1. */
+        for (size_t i = 0; i < slots; i++) ch[i] = cd[i] = cd2[i] = ci[i] = ci2[i] = INF;
+        const int64_t c_first = r - hi > 0 ? r - hi : 0, c_last = r - lo < m ? r - lo : m;
+        for (int64_t c = c_first; c <= c_last; c++) {
+            const int64_t j = c - r + hi;
+            const size_t at = (size_t)j + 1;
+            if (r == 0) {
+                ch[at] = inserted(costs, c);
+                continue;
+            }
+            if (c == 0) {
+                ch[at] = deleted(costs, r);
+                continue;
+            }
+            const int equal = ref[r - 1] == qry[c - 1];
+            const int64_t substituted = ph[at] + (equal ? 0 : costs->mismatch);
+            const int64_t opened_del = ph[at + 1] + costs->del_open + costs->del_ext;
+            const int64_t extended_del = pd[at + 1] + costs->del_ext;
+            const int64_t del = min64(opened_del, extended_del);
+            const int64_t opened_ins = ch[at - 1] + costs->ins_open + costs->ins_ext;
+            const int64_t extended_ins = ci[at - 1] + costs->ins_ext;
+            const int64_t in = min64(opened_ins, extended_ins);
+            int64_t score = min64(min64(substituted, del), in);
+            int64_t opened_del2 = INF, extended_del2 = INF, del2 = INF;
+            int64_t opened_ins2 = INF, extended_ins2 = INF, in2 = INF;
+            if (costs->pieces == 2) {
+                opened_del2 = ph[at + 1] + costs->del_open2 + costs->del_ext2;
+                extended_del2 = pd2[at + 1] + costs->del_ext2;
+                del2 = min64(opened_del2, extended_del2);
+                opened_ins2 = ch[at - 1] + costs->ins_open2 + costs->ins_ext2;
+                extended_ins2 = ci2[at - 1] + costs->ins_ext2;
+                in2 = min64(opened_ins2, extended_ins2);
+                score = min64(score, min64(del2, in2));
+            }
+            if (score >= INF) score = INF;
+            ch[at] = score;
+            cd[at] = min64(del, INF);
+            ci[at] = min64(in, INF);
+            cd2[at] = min64(del2, INF);
+            ci2[at] = min64(in2, INF);
+            if (flags) {
+                uint8_t flag = in == score ? SRC_INS : SRC_ALIGNED;
+                if (costs->pieces == 2 && in2 == score) flag = SRC_INS2;
+                if (del == score) flag = SRC_DEL;
+                if (costs->pieces == 2 && del2 == score) flag = SRC_DEL2;
+                if (!equal && substituted == score) flag = SRC_ALIGNED;
+                if (extended_del <= opened_del) flag |= EXT_BIT(SRC_DEL);
+                if (extended_ins <= opened_ins) flag |= EXT_BIT(SRC_INS);
+                if (costs->pieces == 2) {
+                    if (extended_del2 <= opened_del2) flag |= EXT_BIT(SRC_DEL2);
+                    if (extended_ins2 <= opened_ins2) flag |= EXT_BIT(SRC_INS2);
+                }
+                flags[(size_t)r * (size_t)width + (size_t)j] = flag;
+            }
+        }
+        if (r == n && n - m >= lo && n - m <= hi) corner = ch[(size_t)(m - n + hi) + 1];
+        int64_t *t;
+        t = ph, ph = ch, ch = t;
+        t = pd, pd = cd, cd = t;
+        t = pd2, pd2 = cd2, cd2 = t;
+    }
+    free(mem);
+    return corner;
+}
+
+/* The band every path dearer than nothing but `target` stays off: diagonals from 0 and n-m outwards while a path
+ * through the next one could cost `target` or less (strict) or less than `target` (plain). */
+static void band_for(const orc_costs *costs, int64_t n, int64_t m, int64_t target, int strict, int64_t *lo,
+                     int64_t *hi) {
+    const int64_t e = n - m;
+    *lo = e < 0 ? e : 0;
+    *hi = e > 0 ? e : 0;
+    while (*hi < n && (strict ? beyond(costs, n, m, *hi + 1) <= target : beyond(costs, n, m, *hi + 1) < target))
+        ++*hi;
+    while (*lo > -m && (strict ? beyond(costs, n, m, *lo - 1) <= target : beyond(costs, n, m, *lo - 1) < target))
+        --*lo;
+}
+
+int64_t orc_costs_distance_proven(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m,
+                                  const orc_costs *costs) {
+    /* A first band 8 diagonals past 0 and n-m, then, if a cheaper path could leave it, the band such a path
+     * needs (the plain certificate of lanes.mojo:1337-1392): the second sweep's cost is then the least. */
+    const int64_t e = n - m;
+    int64_t lo = (e < 0 ? e : 0) - 8, hi = (e > 0 ? e : 0) + 8;
+    if (lo < -m) lo = -m;
+    if (hi > n) hi = n;
+    int64_t found = band_sweep(ref, n, qry, m, costs, lo, hi, NULL);
+    const int64_t off = min64(lo > -m ? beyond(costs, n, m, lo - 1) : INF, hi < n ? beyond(costs, n, m, hi + 1) : INF);
+    if (found <= off) return found;
+    band_for(costs, n, m, found, 0, &lo, &hi);
+    return band_sweep(ref, n, qry, m, costs, lo, hi, NULL);
+}
+
+int64_t orc_costs_align_proven(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                               int right_ties, int eqx, char *cigar, int64_t cap, int64_t *band_cells) {
+    const int64_t cost = orc_costs_distance_proven(ref, n, qry, m, costs);
+    /* Every path off the strict band costs more than the least, so the flags inside it are the whole matrix's on
+     * every optimal path; the sweep is over both sequences reversed for Ties.LEFT, whose band is the same set
+     * of diagonals (a bound on any path through a diagonal depends on the lengths alone). */
+    int64_t lo, hi;
+    band_for(costs, n, m, cost, 1, &lo, &hi);
+    const int64_t width = hi - lo + 1;
+    if (band_cells) *band_cells = width * (n + 1);
+    const int left = !right_ties;
+    uint8_t *a = malloc((size_t)n + 1), *b = malloc((size_t)m + 1);
+    for (int64_t i = 0; i < n; i++) a[i] = left ? ref[n - 1 - i] : ref[i];
+    for (int64_t i = 0; i < m; i++) b[i] = left ? qry[m - 1 - i] : qry[i];
+    uint8_t *flags = calloc((size_t)(n + 1) * (size_t)width, 1);
+    if (!flags) {
+        free(a);
+        free(b);
+        return -2;
+    }
+    const int64_t again = band_sweep(a, n, b, m, costs, lo, hi, flags);
+    free(a);
+    free(b);
+    if (again != cost) {
+        free(flags);
+        return -9; /* the band lost the least cost: the certificate is broken */
+    }
+    /* The walk of lanes.mojo:1410-1460, as orc_costs_align walks it. */
+    uint8_t *moves = malloc((size_t)(n + m + 1));
+    int64_t count = 0, r = n, c = m;
+    int layer = SRC_ALIGNED;
+    while (r > 0 && c > 0) {
+        const uint8_t flag = flags[(size_t)r * (size_t)width + (size_t)(c - r + hi)];
+        if (layer == SRC_ALIGNED) {
+            layer = flag & 7;
+            if (layer == SRC_ALIGNED) {
+                moves[count++] = MOVE_ALIGNED;
+                r--;
+                c--;
+            }
+            continue;
+        }
+        const int extended = (flag & EXT_BIT(layer)) != 0;
+        if (layer % 2 == 1) {
+            moves[count++] = MOVE_DEL;
+            r--;
+        } else {
+            moves[count++] = MOVE_INS;
+            c--;
+        }
+        if (!extended) layer = SRC_ALIGNED;
+    }
+    while (r > 0) moves[count++] = MOVE_DEL, r--;
+    while (c > 0) moves[count++] = MOVE_INS, c--;
+    free(flags);
+    Cigar w = {cigar, cap, 0, 0, 0, 0};
+    int64_t ri = 0, qi = 0;
+    for (int64_t i = 0; i < count; i++) {
+        const uint8_t move = left ? moves[i] : moves[count - 1 - i];
+        if (move == MOVE_DEL) {
+            cigar_add(&w, 'D');
+            ri++;
+        } else if (move == MOVE_INS) {
+            cigar_add(&w, 'I');
+            qi++;
+        } else {
+            cigar_add(&w, eqx ? (ref[ri] == qry[qi] ? '=' : 'X') : 'M');
+            ri++;
+            qi++;
+        }
+    }
+    free(moves);
+    const int64_t written = cigar_finish(&w);
+    return written < 0 ? written : cost;
+}
+
+/* ---------------------------------------------------------------- checking an alignment against its pair */
+
+int64_t orc_cigar_cost(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                       const char *cigar, int64_t length) {
+    int64_t r = 0, q = 0, total = 0, count = 0;
+    int digits = 0;
+    for (int64_t i = 0; i < length; i++) {
+        const char ch = cigar[i];
+        if (ch >= '0' && ch <= '9') {
+            count = count * 10 + (ch - '0');
+            digits = 1;
+            continue;
+        }
+        if (!digits || count <= 0) return -1;
+        if (ch == 'D') {
+            if (r + count > n) return -2;
+            total += deleted(costs, count);
+            r += count;
+        } else if (ch == 'I') {
+            if (q + count > m) return -2;
+            total += inserted(costs, count);
+            q += count;
+        } else if (ch == '=' || ch == 'X' || ch == 'M') {
+            if (r + count > n || q + count > m) return -2;
+            for (int64_t k = 0; k < count; k++) {
+                const int same = ref[r + k] == qry[q + k];
+                if ((ch == '=' && !same) || (ch == 'X' && same)) return -3;
+                if (!same) total += costs->mismatch;
+            }
+            r += count;
+            q += count;
+        } else {
+            return -1;
+        }
+        count = 0;
+        digits = 0;
+    }
+    if (digits || r != n || q != m) return -2;
+    return total;
+}
+
+int64_t orc_check_alignments(int64_t pairs, const uint8_t *const *refs, const int64_t *ref_lengths,
+                             const uint8_t *const *qrys, const int64_t *qry_lengths, const orc_costs *costs,
+                             const char *const *cigars, const int64_t *cigar_lengths, const int64_t *claimed,
+                             int64_t *first_bad) {
+    int64_t bad = 0;
+    *first_bad = -1;
+    for (int64_t i = 0; i < pairs; i++) {
+        const int64_t priced = orc_cigar_cost(refs[i], ref_lengths[i], qrys[i], qry_lengths[i], costs, cigars[i],
+                                              cigar_lengths[i]);
+        if (priced < 0 || priced != claimed[i]) {
+            if (*first_bad < 0) *first_bad = i;
+            bad++;
+        }
+    }
+    return bad;
+}

@@ -52,6 +52,29 @@ int64_t orc_costs_cost(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_
 int64_t orc_costs_align(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
                         int64_t band_low, int64_t band_high, int right_ties, int eqx, char *cigar, int64_t cap);
 
+/* orc_costs_cost and orc_costs_align for long pairs, global and unbanded, in memory proportional to a band: the
+ * cost from a first band 8 diagonals wide plus, when the plain certificate (lanes.mojo:1337-1392) cannot prove it,
+ * the band a cheaper path would need; then the flags of the strict proven band alone (every path off it costs
+ * more, so on every optimal path they are the whole matrix's) walked as orc_costs_align walks them. *band_cells
+ * (unless null) gets the traced band's stored cells. -9: the band lost the least cost (never, if the certificate
+ * holds). */
+int64_t orc_costs_distance_proven(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m,
+                                  const orc_costs *costs);
+int64_t orc_costs_align_proven(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                               int right_ties, int eqx, char *cigar, int64_t cap, int64_t *band_cells);
+
+/* What a CIGAR's edits cost under `costs` (each gap run at its cheaper piece), after checking that it spells an
+ * alignment of the whole pair: -1 malformed, -2 it does not consume exactly both sequences, -3 an `=` over
+ * unequal letters or an `X` over equal ones. */
+int64_t orc_cigar_cost(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                       const char *cigar, int64_t length);
+
+/* orc_cigar_cost over a batch against each pair's claimed cost: the pairs that fail, the first in *first_bad. */
+int64_t orc_check_alignments(int64_t pairs, const uint8_t *const *refs, const int64_t *ref_lengths,
+                             const uint8_t *const *qrys, const int64_t *qry_lengths, const orc_costs *costs,
+                             const char *const *cigars, const int64_t *cigar_lengths, const int64_t *claimed,
+                             int64_t *first_bad);
+
 /* The least any path visiting lane diagonal `diagonal` (query less reference) costs, its end on lane
  * diagonal `end`, no free ends: reference lanes.mojo:333-345. */
 int64_t orc_off_band(const orc_costs *costs, int64_t diagonal, int64_t end);

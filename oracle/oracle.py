@@ -56,6 +56,16 @@ def _load() -> ctypes.CDLL:
     lib.orc_scoring_local_align.argtypes = [p8, i64, p8, i64, ctypes.c_void_p, p8, p8, ctypes.c_int,
                                             ctypes.c_void_p, i64, pi64]
     lib.orc_scoring_local_align.restype = i64
+    lib.orc_costs_distance_proven.argtypes = [p8, i64, p8, i64, ctypes.c_void_p]
+    lib.orc_costs_distance_proven.restype = i64
+    lib.orc_costs_align_proven.argtypes = [p8, i64, p8, i64, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p, i64, pi64]
+    lib.orc_costs_align_proven.restype = i64
+    lib.orc_cigar_cost.argtypes = [p8, i64, p8, i64, ctypes.c_void_p, p8, i64]
+    lib.orc_cigar_cost.restype = i64
+    vp = ctypes.c_void_p
+    lib.orc_check_alignments.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, vp, pi64]
+    lib.orc_check_alignments.restype = i64
     del pc
     return lib
 
@@ -146,6 +156,58 @@ def costs_align(ref, qry, costs: Costs, band: Tuple[int, int] = (NO_BAND_LOW, NO
     if cost < 0:
         return cost, None
     return cost, buf.value.decode("ascii")
+
+
+def costs_distance_proven(ref, qry, costs: Costs) -> int:
+    """``costs_cost`` of a global unbanded pair in memory proportional to a band (long pairs)."""
+    a, b = _b(ref), _b(qry)
+    c = costs._c()
+    return lib().orc_costs_distance_proven(a, len(a), b, len(b), ctypes.byref(c))
+
+
+def costs_align_proven(ref, qry, costs: Costs, right_ties: bool = False, eqx: bool = True,
+                       with_cells: bool = False):
+    """``costs_align`` of a global unbanded pair, tracing the strict proven band alone (long pairs). With
+    ``with_cells``, also the stored band's cells."""
+    a, b = _b(ref), _b(qry)
+    c = costs._c()
+    cap = 2 * (len(a) + len(b)) + 2
+    buf = ctypes.create_string_buffer(cap)
+    cells = ctypes.c_int64(0)
+    cost = lib().orc_costs_align_proven(a, len(a), b, len(b), ctypes.byref(c), int(right_ties), int(eqx), buf, cap,
+                                        ctypes.byref(cells))
+    found = (cost, buf.value.decode("ascii") if cost >= 0 else None)
+    return (*found, cells.value) if with_cells else found
+
+
+def cigar_cost(ref, qry, costs: Costs, cigar) -> int:
+    """What `cigar` costs if it spells an alignment of the whole pair, else -1 (malformed), -2 (wrong letter
+    counts) or -3 (an `=` or `X` that the letters contradict)."""
+    a, b = _b(ref), _b(qry)
+    text = _b(cigar)
+    c = costs._c()
+    return lib().orc_cigar_cost(a, len(a), b, len(b), ctypes.byref(c), text, len(text))
+
+
+def check_alignments(refs: Sequence[bytes], qrys: Sequence[bytes], costs: Costs, cigars: Sequence[bytes],
+                     claimed: Sequence[int]) -> Tuple[int, int]:
+    """(failures, first failing index or -1): each CIGAR must spell its pair and price at its claimed cost."""
+    count = len(refs)
+    keep = [_b(r) for r in refs], [_b(q) for q in qrys], [_b(x) for x in cigars]
+    arrays = []
+    for items in keep:
+        ptrs = (ctypes.c_char_p * max(count, 1))(*items)
+        lens = (ctypes.c_int64 * max(count, 1))(*[len(x) for x in items])
+        arrays.append((ptrs, lens))
+    claim = (ctypes.c_int64 * max(count, 1))(*[int(v) for v in claimed])
+    c = costs._c()
+    first = ctypes.c_int64(-1)
+    vp = ctypes.c_void_p
+    bad = lib().orc_check_alignments(count, ctypes.cast(arrays[0][0], vp), ctypes.cast(arrays[0][1], vp),
+                                     ctypes.cast(arrays[1][0], vp), ctypes.cast(arrays[1][1], vp), ctypes.byref(c),
+                                     ctypes.cast(arrays[2][0], vp), ctypes.cast(arrays[2][1], vp),
+                                     ctypes.cast(claim, vp), ctypes.byref(first))
+    return int(bad), int(first.value)
 
 
 def off_band(costs: Costs, diagonal: int, end: int) -> int:

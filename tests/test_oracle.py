@@ -423,3 +423,60 @@ def test_proven_band_traces_the_whole_matrix_alignment(costs):
             assert found == whole[0]
             if strict:
                 assert o.costs_align(a, b, costs, (-high, -low)) == whole
+
+
+# ------------------------------------------------------------------ the banded-memory oracle for long pairs
+
+
+def _random_pairs(count, seed, longest=60):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(count):
+        alphabet = rng.choice(["AC", "ACGT", "A", "ACG"])
+        ref = "".join(rng.choice(alphabet) for _ in range(rng.randint(0, longest)))
+        qry = list(ref)
+        for _ in range(rng.randint(0, 8)):
+            at = rng.randint(0, len(qry))
+            event = rng.random()
+            if event < 0.3 and at < len(qry):
+                del qry[at:at + rng.randint(1, 10)]
+            elif event < 0.6:
+                qry[at:at] = [rng.choice(alphabet) for _ in range(rng.randint(1, 10))]
+            elif at < len(qry):
+                qry[at] = rng.choice(alphabet)
+        out.append((ref, "".join(qry)))
+    return out
+
+
+@pytest.mark.parametrize("costs", [AFFINE, EDIT, TWO, o.Costs.affine(4, 6, 2).with_deletions(3, 1),
+                                   o.Costs.two_piece(2, 2, 2, 4, 1)])
+def test_banded_memory_oracle_equals_the_whole_matrix_oracle(costs):
+    # The long-pair oracle stores the strict proven band alone; on every pair it must give the whole matrix's cost
+    # and CIGAR under both tie rules, and its distance the whole matrix's.
+    for ref, qry in _random_pairs(500, seed=91):
+        for right in (False, True):
+            assert o.costs_align_proven(ref, qry, costs, right_ties=right) == \
+                o.costs_align(ref, qry, costs, right_ties=right), (ref, qry, right)
+        assert o.costs_distance_proven(ref, qry, costs) == o.costs_cost(ref, qry, costs)
+
+
+def test_banded_memory_oracle_stores_a_band_not_the_matrix():
+    rng = random.Random(92)
+    ref = "".join(rng.choice("ACGT") for _ in range(3000))
+    qry = ref[:1000] + ref[1010:2000] + "ACGTA" + ref[2000:]
+    cost, cigar, cells = o.costs_align_proven(ref, qry, EDIT, with_cells=True)
+    assert (cost, cigar) == o.costs_align(ref, qry, EDIT)
+    assert cells < 3001 * 40  # the strict band is a few dozen diagonals, not 6,000
+
+
+def test_cigar_checker_rejects_what_does_not_spell_the_pair():
+    ref, qry = "ACGTACGTTTGCA", "ACGTCGTTTTGCA"
+    assert o.cigar_cost(ref, qry, AFFINE, "4=3X6=") == 12
+    assert o.cigar_cost(ref, qry, EDIT, "4=1D2=1I6=") == 2
+    assert o.cigar_cost(ref, qry, TWO, "4=1D2=1I6=") == 16
+    assert o.cigar_cost(ref, qry, AFFINE, "4=3X5=") == -2   # a letter short
+    assert o.cigar_cost(ref, qry, AFFINE, "4=3=6=") == -3   # `=` over unequal letters
+    assert o.cigar_cost(ref, qry, AFFINE, "5=2X6=") == -3   # ... and over a wrong split
+    assert o.cigar_cost(ref, qry, AFFINE, "4=3X6") == -2    # a count with no operation
+    assert o.cigar_cost(ref, qry, AFFINE, "4=3Q6=") == -1
+    assert o.check_alignments([ref, ref], [qry, qry], AFFINE, ["4=3X6=", "4=3X6="], [12, 13]) == (1, 1)
