@@ -570,6 +570,87 @@ def timed_steps(handle, b, steps, warmup, before=None):
     return float(np.mean(walls)), float(np.mean(device))
 
 
+def end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, chosen, floor, use_graph):
+    """The learned path end to end on a fresh batch, as a caller holding the sequences in host memory meets it: pack
+    and upload (dinara_batch_new), learned seeding and the run (the first run of a batch: its graph captured, its
+    arena reserved), and the alignments copied out with their CIGARs malloc'ed (dinara_batch_alignments). Then the
+    drop-in call alone (dinara_gpu_alignments: the default persistent path, no seeds) on the same pairs."""
+    count = len(refs)
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    costs = _lib.CCosts(1, 0, 1, -1, 0, 0, 0, -1, 0)
+    status = ctypes.c_int64(0)
+    found = (_lib.CAlignment * count)()
+    statuses = (ctypes.c_int64 * count)()
+
+    def release():
+        for i in range(count):
+            if statuses[i] == 0:
+                handle.dinara_free(found[i].cigar)
+
+    t0 = time.perf_counter()
+    b = handle.dinara_batch_new(0, _lib.BATCH_ALIGNMENTS, count, *R.args(), *Q.args(), ctypes.byref(costs), None,
+                                None, workers, ctypes.byref(status))
+    if not b:
+        raise RuntimeError(f"dinara_batch_new failed: {status.value}")
+    t1 = time.perf_counter()
+    handle.dinara_batch_set_tuning(b, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
+    handle.dinara_batch_set_side_by_side(b, chosen[6])
+    handle.dinara_batch_set_keep_first(b, 1)
+    handle.dinara_batch_set_graph(b, use_graph)
+    if handle.dinara_batch_seed_learned_by_length(b, net, ctypes.c_double(margin), ctypes.c_double(slope)) != 0 or \
+            handle.dinara_batch_run(b) != 0:
+        raise RuntimeError("end-to-end run failed")
+    t2 = time.perf_counter()
+    if handle.dinara_batch_alignments(b, found, statuses) != 0:
+        raise RuntimeError("end-to-end fetch failed")
+    t3 = time.perf_counter()
+    release()
+    handle.dinara_batch_free(b)
+    t4 = time.perf_counter()
+    if handle.dinara_gpu_alignments(0, count, *R.args(), *Q.args(), ctypes.byref(costs), None, None, workers, found,
+                                    statuses) != 0:
+        raise RuntimeError("drop-in call failed")
+    t5 = time.perf_counter()
+    release()
+    total = t3 - t0
+    return {"pairs_per_s": round(count / total, 1), "ms": round(total * 1e3, 1),
+            "pack_upload_ms": round((t1 - t0) * 1e3, 1), "seed_and_run_ms": round((t2 - t1) * 1e3, 1),
+            "fetch_ms": round((t3 - t2) * 1e3, 1),
+            "drop_in_call_pairs_per_s": round(count / (t5 - t4), 1), "drop_in_call_ms": round((t5 - t4) * 1e3, 1),
+            "note": "learned path on a fresh batch: pack + upload, seed + first run (graph capture, arena), copy out "
+                    "with CIGARs malloc'ed; the drop-in dinara_gpu_alignments call (default path, no seeds) beside it"}
+
+
+def drop_in_call(handle, R, Q, count, costs, options, threads, reps=3, kind=_lib.BATCH_ALIGNMENTS):
+    """The end-to-end rate of the drop-in call (dinara_gpu_alignments, or dinara_gpu_distances): validation, packing,
+    upload, the run, the copy back and (alignments) each CIGAR malloc'ed, for pairs already in host memory; the
+    mean of `reps` calls after an untimed one. Freeing the CIGARs is the caller's and not timed."""
+    found = (_lib.CAlignment * count)()
+    statuses = (ctypes.c_int64 * count)()
+    results = (ctypes.c_int64 * count)()
+    walls = []
+    for rep in range(reps + 1):
+        t0 = time.perf_counter()
+        if kind == _lib.BATCH_ALIGNMENTS:
+            code = handle.dinara_gpu_alignments(0, count, *R.args(), *Q.args(), costs, None, options, threads, found,
+                                                statuses)
+        else:
+            code = handle.dinara_gpu_distances(0, count, *R.args(), *Q.args(), costs, None, options, threads, results)
+        wall = time.perf_counter() - t0
+        if code != 0:
+            raise RuntimeError(f"drop-in call failed: {code}")
+        if kind == _lib.BATCH_ALIGNMENTS:
+            for i in range(count):
+                if statuses[i] == 0:
+                    handle.dinara_free(found[i].cigar)
+        if rep:
+            walls.append(wall)
+    ms = float(np.mean(walls) * 1e3)
+    return {"pairs_per_s": round(count / (ms / 1e3), 1), "ms": round(ms, 2), "threads": threads,
+            "note": "the whole drop-in call on host-resident pairs: pack, upload, run, copy back"
+                    + (", CIGARs malloc'ed" if kind == _lib.BATCH_ALIGNMENTS else "")}
+
+
 def bin_profile(handle, b):
     """One run launched kernel by kernel (not the graph): each bin's first-pass and tracing time and sweeps."""
     handle.dinara_batch_set_graph(b, 0)
@@ -600,13 +681,18 @@ def run_uneven(args) -> int:
     n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
     m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
     print(f"[bench] uneven workload ready: {pairs} pairs in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    # The training pairs are made here too, before the first GPU call, so the run's memory peak comes first.
-    train_r = train_q = None
-    if train_n > 0 and not args.plain_only:
+    # The training pairs and the held-out tuning pairs are made here too, before the first GPU call (no process is
+    # forked after it). Every launch choice and the seeds' margin and slope are chosen by timing the tuning pairs
+    # (drawn like the workload, another seed) and frozen before the workload's pairs are timed.
+    train_r = train_q = tune_r = tune_q = None
+    if not args.plain_only:
         t0 = time.time()
-        train_r, train_q = synthetic.uneven_parallel(train_n, seed=41, workers=workers)
-        print(f"[bench] training pairs ready: {train_n} pairs in {time.time() - t0:.1f}s", file=sys.stderr,
-              flush=True)
+        tune_n = args.tune_pairs or pairs
+        tune_r, tune_q = synthetic.uneven_parallel(tune_n, seed=44, workers=workers)
+        if train_n > 0:
+            train_r, train_q = synthetic.uneven_parallel(train_n, seed=41, workers=workers)
+        print(f"[bench] tuning ({tune_n}) and training ({train_n}) pairs ready in {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     # Plain doubling is also timed in a child process at HIP's default 4 hardware queues (more queues let the
     # bins run at once, which helps the other variants and can slow this one): the baseline keeps its better time.
     child = None
@@ -663,34 +749,32 @@ def run_uneven(args) -> int:
     print(f"[bench] plain doubling {plain_ms:.1f} ms/step ({plain_by_queues} by hardware queues)", file=sys.stderr,
           flush=True)
 
-    # Persistent, captured, autotuned.
-    batch, keep = edit_batch(handle, refs, qrys, A, workers)
+    # Every choice below is made on the held-out tuning pairs and frozen; the workload's pairs are then timed once
+    # with those choices.
+    # Persistent, captured: the autotuner's launch shape (workgroups a compute unit, start bins, bins side by side).
+    tb, tkeep = edit_batch(handle, tune_r, tune_q, A, workers)
     chosen = (ctypes.c_int64 * 7)()
     t0 = time.time()
-    if handle.dinara_batch_autotune(batch, chosen) != 0:
+    if handle.dinara_batch_autotune(tb, chosen) != 0:
         raise RuntimeError("autotune failed")
     tune_s = time.time() - t0
-    persistent_ms, persistent_dev = timed_steps(handle, batch, args.steps, args.warmup)
-    results["persistent"] = edit_results(handle, batch, A, pairs)
-    prof = bin_profile(handle, batch)
-    first_proven = handle.dinara_batch_stat(batch, 4)
-    print(f"[bench] persistent {persistent_ms:.1f} ms/step (tuned in {tune_s:.1f}s: {list(chosen)})",
-          file=sys.stderr, flush=True)
+    print(f"[bench] persistent shape tuned on held-out pairs in {tune_s:.1f}s: {list(chosen)}", file=sys.stderr,
+          flush=True)
 
     # Learned seeds: a network trained on the aligner's own distances of separate pairs.
-    learned_ms = learned_dev = None
+    net = None
     learned_info = None
     if train_n > 0:
         t0 = time.time()
-        tb, tkeep = edit_batch(handle, train_r, train_q, _lib.BATCH_DISTANCES, workers)
-        if handle.dinara_batch_run(tb) != 0:
+        lb, lkeep = edit_batch(handle, train_r, train_q, _lib.BATCH_DISTANCES, workers)
+        if handle.dinara_batch_run(lb) != 0:
             raise RuntimeError("labelling run failed")
-        labels, _ = edit_results(handle, tb, _lib.BATCH_DISTANCES, train_n)
+        labels, _ = edit_results(handle, lb, _lib.BATCH_DISTANCES, train_n)
         x_train = np.zeros((train_n, 288), dtype=np.float32)
-        if handle.dinara_batch_features(tb, x_train.ctypes.data) != 0:
+        if handle.dinara_batch_features(lb, x_train.ctypes.data) != 0:
             raise RuntimeError("features failed")
-        handle.dinara_batch_free(tb)
-        del tkeep, train_r, train_q
+        handle.dinara_batch_free(lb)
+        del lkeep, train_r, train_q
         y_train = np.log1p(np.maximum(labels, 0)).astype(np.float32)
         net = handle.dinara_mlp_new(0, 17)
         loss = ctypes.c_double(0)
@@ -700,75 +784,104 @@ def run_uneven(args) -> int:
                                    ctypes.byref(loss)) != 0:
             raise RuntimeError("training failed")
         train_s = time.time() - t1
+        label_train_s = time.time() - t0
         del x_train
+
+        def seeder(b, margin, slope):
+            return lambda: handle.dinara_batch_seed_learned_by_length(b, net, ctypes.c_double(margin),
+                                                                      ctypes.c_double(slope))
+
+        # On the tuning pairs: sweeps keep their decisions (a proven first band is walked at once), the launch shape
+        # tuned again for the seeds, then margin and slope, first-bin floors and graph or not, each timed.
         margin, slope = args.margin, args.slope
-        seed = lambda: handle.dinara_batch_seed_learned_by_length(batch, net, ctypes.c_double(margin),
-                                                                  ctypes.c_double(slope))
-        # The seeds choose each pair's first band: no start bin holds it wider.
-        handle.dinara_batch_set_tuning(batch, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
-        seed()
-        two_pass_ms, _ = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
-        results["learned_two_pass"] = edit_results(handle, batch, A, pairs)
-        # With seeds that mostly prove their pairs, sweeps keep their decisions: no second sweep. The launch shape
-        # (workgroups, bins side by side or not) is tuned again for it.
-        handle.dinara_batch_set_keep_first(batch, 1)
-        seed()
+        handle.dinara_batch_set_tuning(tb, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
+        handle.dinara_batch_set_keep_first(tb, 1)
+        seeder(tb, margin, slope)()
         learned_chosen = (ctypes.c_int64 * 7)()
-        if handle.dinara_batch_autotune(batch, learned_chosen) != 0:
+        if handle.dinara_batch_autotune(tb, learned_chosen) != 0:
             raise RuntimeError("autotune failed")
-        # The seeds' margin and slope, timed on the batch (like the launch shape): the fastest pair is kept.
         by_slope = {}
         if args.slopes or args.margins:
             slopes = [float(v) for v in args.slopes.split(",")] if args.slopes else [args.slope]
             margins = [float(v) for v in args.margins.split(",")] if args.margins else [args.margin]
-            for margin in margins:
-                for slope in slopes:
-                    by_slope[(margin, slope)] = timed_steps(handle, batch, 2, 1, before=seed)[0]
-                    print(f"[bench] margin {margin} slope {slope}: {by_slope[(margin, slope)]:.1f} ms",
-                          file=sys.stderr, flush=True)
+            for mg in margins:
+                for sl in slopes:
+                    by_slope[(mg, sl)] = timed_steps(handle, tb, 2, 1, before=seeder(tb, mg, sl))[0]
+                    print(f"[bench] held-out: margin {mg} slope {sl}: {by_slope[(mg, sl)]:.1f} ms", file=sys.stderr,
+                          flush=True)
             margin, slope = min(by_slope, key=by_slope.get)
-        # Least first bins for the two longest length classes (a long pair whose first band fails sweeps its whole
-        # length again, the run's longest chain), timed the same way.
         by_floor = {}
         floor = (0, 0)
         if args.floors:
             for item in args.floors.split(","):
                 f3, f4 = (int(v) for v in item.split(":"))
-                handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, f3, f4))
-                by_floor[(f3, f4)] = timed_steps(handle, batch, 2, 1, before=seed)[0]
-                print(f"[bench] first-bin floors {f3}:{f4}: {by_floor[(f3, f4)]:.1f} ms", file=sys.stderr, flush=True)
+                handle.dinara_batch_set_tuning(tb, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, f3, f4))
+                by_floor[(f3, f4)] = timed_steps(handle, tb, 2, 1, before=seeder(tb, margin, slope))[0]
             floor = min(by_floor, key=by_floor.get)
-            handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
-        # One captured graph a run, or the same launches one by one: timed on the batch, the faster kept.
+            handle.dinara_batch_set_tuning(tb, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
         by_launch = {}
         for graph in (1, 0):
-            handle.dinara_batch_set_graph(batch, graph)
-            by_launch[graph] = timed_steps(handle, batch, 3, 1, before=seed)[0]
-            print(f"[bench] learned {'graph' if graph else 'launches one by one'}: {by_launch[graph]:.1f} ms",
+            handle.dinara_batch_set_graph(tb, graph)
+            by_launch[graph] = timed_steps(handle, tb, 3, 1, before=seeder(tb, margin, slope))[0]
+            print(f"[bench] held-out: learned {'graph' if graph else 'launches one by one'}: {by_launch[graph]:.1f} ms",
                   file=sys.stderr, flush=True)
         use_graph = min(by_launch, key=by_launch.get)
+    handle.dinara_batch_free(tb)
+    del tkeep, tune_r, tune_q
+
+    # The workload's pairs, the persistent path with the frozen shape.
+    batch, keep = edit_batch(handle, refs, qrys, A, workers)
+    handle.dinara_batch_set_tuning(batch, chosen[0], (ctypes.c_int64 * 5)(*[chosen[i] for i in range(1, 6)]))
+    handle.dinara_batch_set_side_by_side(batch, chosen[6])
+    persistent_ms, persistent_dev = timed_steps(handle, batch, args.steps, args.warmup)
+    results["persistent"] = edit_results(handle, batch, A, pairs)
+    prof = bin_profile(handle, batch)
+    first_proven = handle.dinara_batch_stat(batch, 4)
+    print(f"[bench] persistent {persistent_ms:.1f} ms/step", file=sys.stderr, flush=True)
+
+    # The learned path with the frozen choices: seeding (features, prediction, first bands) inside each step.
+    learned_ms = learned_dev = None
+    end_to_end = None
+    if net is not None:
+        handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
+        handle.dinara_batch_set_side_by_side(batch, learned_chosen[6])
         handle.dinara_batch_set_graph(batch, use_graph)
+        seed = seeder(batch, margin, slope)
+        seed()
+        two_pass_ms, _ = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
+        results["learned_two_pass"] = edit_results(handle, batch, A, pairs)
+        handle.dinara_batch_set_keep_first(batch, 1)
         learned_ms, learned_dev = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
         results["learned"] = edit_results(handle, batch, A, pairs)
         learned_proven = handle.dinara_batch_stat(batch, 4)
-        walked_first = handle.dinara_batch_stat(batch, 9)
+        walked_kept = handle.dinara_batch_stat(batch, 9)
         learned_prof = bin_profile(handle, batch)
+        handle.dinara_batch_free(batch)
+        del keep
+        # End to end: packing and upload, seeding, the run, and the alignments copied out (CIGARs malloc'ed).
+        end_to_end = end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, learned_chosen, floor,
+                                        use_graph)
         learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
-                        "label_and_train_s": round(time.time() - t0, 1), "train_s": round(train_s, 1),
+                        "label_and_train_s": round(label_train_s, 1), "train_s": round(train_s, 1),
+                        "chosen_on": f"{args.tune_pairs or pairs} held-out pairs drawn like the workload (seed 44)",
                         "margin": margin, "slope": slope,
-                        "by_margin_slope_ms": {f"{k[0]},{k[1]}": round(v, 3) for k, v in by_slope.items()},
+                        "by_margin_slope_ms_held_out": {f"{k[0]},{k[1]}": round(v, 3) for k, v in by_slope.items()},
                         "first_bin_floors": list(floor), "graph": int(use_graph),
-                        "by_launch_ms": {("graph" if k else "one_by_one"): round(v, 3) for k, v in by_launch.items()},
-                        "by_floors_ms": {f"{k[0]}:{k[1]}": round(v, 3) for k, v in by_floor.items()},
+                        "by_launch_ms_held_out": {("graph" if k else "one_by_one"): round(v, 3)
+                                                  for k, v in by_launch.items()},
+                        "by_floors_ms_held_out": {f"{k[0]}:{k[1]}": round(v, 3) for k, v in by_floor.items()},
                         "first_band_proved": round(learned_proven / pairs, 4),
-                        "walked_without_second_sweep": round(walked_first / pairs, 4),
+                        "proved_by_a_sweep_keeping_decisions": round(walked_kept / pairs, 4),
                         "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
                         "tuning": {"grid_per_cu": int(learned_chosen[0]), "side_by_side": int(learned_chosen[6])}}
         handle.dinara_mlp_free(net)
         print(f"[bench] learned {learned_ms:.1f} ms/step", file=sys.stderr, flush=True)
-    handle.dinara_batch_free(batch)
-    del keep
+    else:
+        handle.dinara_batch_free(batch)
+        del keep
 
+    launch_form = ("one captured hipGraph a run" if use_graph else "launches one by one") if net is not None \
+        else "one captured hipGraph a run"
     costs, cigars = results["persistent"]
     same_variants = all(np.array_equal(r[0], costs) and r[1] == cigars for r in results.values())
     best_name = min((k for k in ("persistent", "learned") if k in results),
@@ -783,18 +896,26 @@ def run_uneven(args) -> int:
     if learned_info is not None:
         sweeps_ms = learned_info["bins"]["run_ms"] - learned_info["bins"]["walk_ms"]
         kernel = "edit_sweep<keep decisions> (every bin, side by side), learned seeds"
-        ops_per_word = BIT_OPS_TRACE
     else:
         sweeps_ms = prof["run_ms"] - prof["walk_ms"]
         kernel = "edit_sweep (first pass and tracing pass, every bin)"
-        ops_per_word = BIT_OPS_FIRST + BIT_OPS_TRACE
-    achieved = float(proven.sum()) * ops_per_word / 64 / (sweeps_ms / 1e3) / 1e12
+    achieved = float(proven.sum()) * BIT_OPS_FIRST / 64 / (sweeps_ms / 1e3) / 1e12
+    achieved_trace = float(proven.sum()) * BIT_OPS_TRACE / 64 / (sweeps_ms / 1e3) / 1e12
+    peaks = measured_peaks()
+    traffic = MEASURED_TRAFFIC.get(("uneven", pairs, "edit_sweep"))
     roofline = {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": VALU_PEAK_TOPS,
-                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                "traffic": round(traffic["read"] + traffic["written"]) if traffic else None,
                 "launch_ms": round(sweeps_ms, 3),
-                "note": "proven-band cells at the true cost (strict) x 56 int32 ops a 64-cell word (the bit-parallel "
-                        "step and its 2-bit decisions); the sweeps' elapsed time in a run launched kernel by kernel, "
-                        "less the walk"}
+                "achieved_with_decisions": round(achieved_trace, 3),
+                "frac_with_decisions": round(achieved_trace / VALU_PEAK_TOPS, 4),
+                "measured_peak": peaks["valu_int32_tops"],
+                "frac_of_measured_peak": round(achieved / peaks["valu_int32_tops"], 4) if peaks["valu_int32_tops"]
+                else None,
+                "note": "proven-band cells at the true cost (strict) x 34 int32 ops a 64-cell word (the bit-parallel "
+                        "step); _with_decisions counts 56 (the step and its 2-bit decisions); the sweeps' elapsed "
+                        "time in a run launched kernel by kernel, less the walk; traffic: HBM bytes the learned run's "
+                        "sweeps moved, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (calibrated on a plain copy)"}
     # The same sweeps against HBM: the 2-bit decisions the walk needs (proven-band cells, algorithmic), and those
     # the sweeps write (every word-column swept keeping decisions, 16 bytes each).
     bins_now = learned_info["bins"] if learned_info is not None else prof
@@ -849,7 +970,7 @@ def run_uneven(args) -> int:
         "config": {
             "workload": "unit-cost distance + Ties.LEFT CIGAR, 200,000 pairs log-normal around 1.5 kbp (sigma 1, "
                         "100 bp..20 kbp), 1-15% error (50/25/25), learned band seeds + persistent length-binned "
-                        "hipGraph launch, start bins and workgroups autotuned",
+                        f"launch ({launch_form}), launch shape, margin and slope chosen on held-out pairs",
             "pairs": pairs,
         },
         "gcups": round(float((n * m).sum()) / (value_ms / 1e3) / 1e9, 2),
@@ -872,6 +993,8 @@ def run_uneven(args) -> int:
         "plain_doubling_ms_by_hw_queues": plain_by_queues,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "end_to_end": end_to_end,
+        "measured_peaks": peaks,
     }
     print(json.dumps(line))
     return 0 if same_variants else 1
@@ -896,6 +1019,9 @@ def run_edit(args) -> int:
     costs, _ = edit_results(handle, batch, D, pairs)
     prof = bin_profile(handle, batch)
     handle.dinara_batch_free(batch)
+    unit = _lib.CCosts(1, 0, 1, -1, 0, 0, 0, -1, 0)
+    end_to_end = drop_in_call(handle, *keep, pairs, ctypes.byref(unit), None, min(cores, 16), reps=10,
+                              kind=_lib.BATCH_DISTANCES)
     proven = proven_band_cells(n, m, costs, False, opening=0, extension=1)
     first_ms = float(np.sum(prof["sweep_ms"]))
     achieved = float(proven.sum()) * BIT_OPS_FIRST / 64 / (first_ms / 1e3) / 1e12
@@ -940,8 +1066,107 @@ def run_edit(args) -> int:
         "bins": prof,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "end_to_end": end_to_end,
     }
     print(json.dumps(line))
+    return 0
+
+
+def load_probe():
+    path = os.path.join(ROOT, "dinara-align_amd", "probe", "libdinara_probe.so")
+    lib = ctypes.CDLL(path)
+    lib.probe_valu.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    lib.probe_valu.restype = ctypes.c_double
+    lib.probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    lib.probe_copy.restype = ctypes.c_double
+    return lib
+
+
+def measured_peaks():
+    """The denominators measured on this GPU: int32 VALU ops a second (8 independent add/xor chains a lane, 8
+    workgroups of 256 a compute unit) and a plain 1 GiB device copy's HBM rate (read plus write)."""
+    lib = load_probe()
+    ops = ctypes.c_double(0)
+    valu = lib.probe_valu(8, 4096, 5, ctypes.byref(ops))
+    copy_ms = ctypes.c_double(0)
+    copy = lib.probe_copy(1 << 30, 10, ctypes.byref(copy_ms))
+    return {"valu_int32_tops": round(valu, 2), "hbm_copy_gbs": round(copy, 1),
+            "note": "probe/probe.hip: int32 add/xor chains on every compute unit; a 1 GiB copy kernel, 16 B a lane"}
+
+
+def train_network(handle, refs, qrys, workers, epochs, net_file=""):
+    """A network trained on the aligner's own unit-cost distances of `refs`/`qrys` (log(1 + cost) targets), or the
+    weights kept in `net_file` (an .npy of float32) when it exists; written there after training when named."""
+    net = handle.dinara_mlp_new(0, 17)
+    count = handle.dinara_mlp_parameter_count(net)
+    if net_file and os.path.exists(net_file):
+        weights = np.load(net_file, allow_pickle=False).astype(np.float32)
+        if weights.size != count or handle.dinara_mlp_set_weights(net, weights.ctypes.data) != 0:
+            raise RuntimeError(f"{net_file} does not hold this network's weights")
+        return net, None
+    b, keep = edit_batch(handle, refs, qrys, _lib.BATCH_DISTANCES, workers)
+    if handle.dinara_batch_run(b) != 0:
+        raise RuntimeError("labelling run failed")
+    labels, _ = edit_results(handle, b, _lib.BATCH_DISTANCES, len(refs))
+    x = np.zeros((len(refs), 288), dtype=np.float32)
+    if handle.dinara_batch_features(b, x.ctypes.data) != 0:
+        raise RuntimeError("features failed")
+    handle.dinara_batch_free(b)
+    del keep
+    y = np.log1p(np.maximum(labels, 0)).astype(np.float32)
+    loss = ctypes.c_double(0)
+    if handle.dinara_mlp_train(net, len(refs), x.ctypes.data, y.ctypes.data, epochs, 4096, 2e-3, 7,
+                               ctypes.byref(loss)) != 0:
+        raise RuntimeError("training failed")
+    if net_file:
+        weights = np.zeros(count, dtype=np.float32)
+        handle.dinara_mlp_get_weights(net, weights.ctypes.data)
+        np.save(net_file, weights, allow_pickle=False)
+    return net, loss.value
+
+
+def run_learned_profile(args) -> int:
+    """The learned path's run alone, for profilers: the mixed-length workload's 200,000 pairs seeded by the network
+    (trained as the default bench trains it, or loaded from --net-file), sweeps keeping their decisions, launches
+    one by one so each bin's sweeps are kernels of their own in a trace; `--steps` runs after the training. A plain
+    1 GiB device copy (probe/probe.hip) runs first: the copy the HBM byte counters are calibrated against. Prints the
+    library's own per-bin times of the last run as one JSON line."""
+    pairs = args.pairs or 200_000
+    workers = min(host_cores(), 16)
+    refs, qrys = synthetic.uneven_parallel(pairs, seed=3, workers=workers)
+    train_r = train_q = None
+    if not (args.net_file and os.path.exists(args.net_file)):
+        train_r, train_q = synthetic.uneven_parallel(args.train or 1_000_000, seed=41, workers=workers)
+    print("[profile] pairs ready", file=sys.stderr, flush=True)
+    handle = _lib.lib()
+    copy_ms = ctypes.c_double(0)
+    copy = load_probe().probe_copy(1 << 30, 2, ctypes.byref(copy_ms))
+    print(f"[profile] calibration copy: 1 GiB in {copy_ms.value:.3f} ms ({copy:.0f} GB/s)", file=sys.stderr,
+          flush=True)
+    net, loss = train_network(handle, train_r, train_q, workers, args.epochs or 3, args.net_file)
+    del train_r, train_q
+    print(f"[profile] network ready (loss {loss})", file=sys.stderr, flush=True)
+    b, keep = edit_batch(handle, refs, qrys, _lib.BATCH_ALIGNMENTS, workers)
+    handle.dinara_batch_set_tuning(b, args.grid_per_cu, (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
+    handle.dinara_batch_set_keep_first(b, 1)
+    handle.dinara_batch_set_graph(b, 0)
+    for step in range(args.steps):
+        if handle.dinara_batch_seed_learned_by_length(b, net, ctypes.c_double(args.margin),
+                                                      ctypes.c_double(args.slope)) != 0 or \
+                handle.dinara_batch_run(b) != 0:
+            raise RuntimeError("learned run failed")
+        print(f"[profile] learned run {step + 1}: {handle.dinara_batch_stat(b, 0):.2f} ms", file=sys.stderr,
+              flush=True)
+    prof = {k: [round(handle.dinara_batch_stat(b, base + i), 4) for i in range(EDIT_BINS)]
+            for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900),
+                            ("sweep_word_columns", 1000), ("trace_word_columns", 1100))}
+    prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
+    prof["run_ms"] = handle.dinara_batch_stat(b, 0)
+    prof["first_band_proved"] = handle.dinara_batch_stat(b, 4) / pairs
+    handle.dinara_batch_free(b)
+    handle.dinara_mlp_free(net)
+    print(json.dumps({"profile": "learned run, mixed-length workload", "margin": args.margin, "slope": args.slope,
+                      "grid_per_cu": args.grid_per_cu, "bins": prof}))
     return 0
 
 
@@ -966,6 +1191,15 @@ def main() -> int:
     ap.add_argument("--slope", type=float, default=0.0, help="uneven: the seeds' margin per e-fold of length")
     ap.add_argument("--slopes", default="0,0.1,0.2",
                     help="uneven: comma-separated slopes to time on the batch, keeping the fastest ('' for --slope)")
+    ap.add_argument("--tune-pairs", type=int, default=None,
+                    help="uneven: held-out pairs the launch choices, margin and slope are timed on (default: as many "
+                         "as the workload)")
+    ap.add_argument("--profile-learned", action="store_true",
+                    help="run only the learned path's runs of the mixed-length workload (launches one by one), for "
+                         "rocprofv3; --margin, --slope and --grid-per-cu fix its choices")
+    ap.add_argument("--grid-per-cu", type=int, default=8, help="--profile-learned: persistent workgroups a CU")
+    ap.add_argument("--net-file", default="", help="--profile-learned: load the network from this .npy, or train "
+                                                      "and save it there")
     ap.add_argument("--plain-only", default="", help=argparse.SUPPRESS)  # the child timing plain doubling
     ap.add_argument("--floors", default="",
                     help="uneven: least first bins (4k-16k letters : 16k and more) timed for the learned seeds, e.g. "
@@ -977,6 +1211,8 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (pairs would shard by length-balanced "
               "ranges, one process per GPU); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.profile_learned:
+        return run_learned_profile(args)
     if args.workload == "local":
         return run_local(args)
     if args.workload == "learned":
@@ -1042,6 +1278,9 @@ def main() -> int:
     for i in range(pairs):
         handle.dinara_free(found[i].cigar)
     handle.dinara_batch_free(batch)
+    end_to_end = drop_in_call(handle, R, Q, pairs, ctypes.byref(costs), ctypes.byref(options), min(cores, 16))
+    print(f"[bench] drop-in dinara_gpu_alignments end to end: {end_to_end['ms']:.1f} ms "
+          f"({end_to_end['pairs_per_s']:.0f} pairs/s)", file=sys.stderr, flush=True)
 
     ms_per_step = float(np.mean(walls) * 1e3)
     value = pairs / (ms_per_step / 1e3)
@@ -1139,6 +1378,7 @@ def main() -> int:
                                    if mean_class_ms[c] > 0}, "walk_bands": round(float(np.mean(walk_ms)), 4)},
         "pairs_per_class": {CLASS_NAMES[c]: int(class_pairs[c]) for c in range(CLASSES) if class_pairs[c] > 0},
         "pack_upload_s": round(pack_s, 3),
+        "end_to_end": end_to_end,
     }
     print(json.dumps(line))
     return 0

@@ -1,4 +1,5 @@
-"""Builds libdinara_hip.so, the MI355X library, in-tree with hipcc for gfx950.
+"""Builds libdinara_hip.so, the MI355X library, in-tree with hipcc for gfx950 (and probe/libdinara_probe.so, the
+roofline probes bench.py measures its denominators with).
 
 Each HIP source under csrc/ compiles to an object in parallel, then one shared library links them, placed in
 the Python package folder beside the ctypes binding that loads it. Always compiles: a build is the check that
@@ -16,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIBRARY = os.path.join(HERE, "dinara_align", "libdinara_hip.so")
+PROBE_SOURCE = os.path.join(HERE, "probe", "probe.hip")
+PROBE = os.path.join(HERE, "probe", "libdinara_probe.so")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -57,6 +60,14 @@ def build(jobs: int = 8) -> str:
         sys.stderr.write(result.stdout + result.stderr)
         raise RuntimeError("linking libdinara_hip.so failed")
     os.replace(tmp, LIBRARY)
+    # The roofline probes (bench.py only): a library of their own.
+    tmp = PROBE + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", "-o", tmp, PROBE_SOURCE]
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode != 0:
+        sys.stderr.write(result.stdout + result.stderr)
+        raise RuntimeError("building libdinara_probe.so failed")
+    os.replace(tmp, PROBE)
     return LIBRARY
 
 
