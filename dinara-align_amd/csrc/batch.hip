@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/dinara.h"
+#include "carved.h"
 #include "common.h"
 #include "device.h"
 #include "edit.h"
@@ -198,6 +199,7 @@ struct dinara_batch {
     bool persistent = true;               // pair groups refill from their queue; else one pair a group
     bool keep_first = false;              // alignments: every sweep keeps its decisions (one pass when proven)
     bool side_by_side_bins = true;        // the bins' first queues (and tracing sweeps) on streams of their own
+    bool timed = true;                    // launches made one by one record an event pair each
     int grid_per_cu = 8;                  // persistent workgroups a compute unit
     int start_bin[EDIT_LENGTH_CLASSES] = {};  // the least first bin, by the pair's longer side
     std::vector<int32_t> edit_first_low;  // per slot: its first band's lowest diagonal
@@ -671,7 +673,7 @@ constexpr int EARLY_WALK = 5 * EDIT_BINS;
 // longest chains, and the hardware queues take the streams in order.
 template <typename Use, typename Launch>
 void side_by_side(DeviceContext *ctx, int mark_base, bool timed, int ev, bool apart, Use use, Launch launch) {
-    (void)hipEventRecord(ctx->mark(mark_base), ctx->stream);
+    if (apart) (void)hipEventRecord(ctx->mark(mark_base), ctx->stream);
     for (int k = 0; k < EDIT_BINS; ++k) {
         const int bin = apart ? EDIT_BINS - 1 - k : k;
         if (!use(bin)) continue;
@@ -792,7 +794,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
         b->graph_used = 1;
     } else {
         if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
-        if (const int64_t code = enqueue_unit(b, a, true, trace)) return code;
+        if (const int64_t code = enqueue_unit(b, a, b->timed, trace)) return code;
     }
     if (hipEventRecord(ctx->sweep_stop, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
     // Pairs whose decisions found no room: trace them in further passes, the arena emptied each time.
@@ -827,7 +829,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
     b->stats_pending = true;
     b->stats_slots = slots;
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
-        if (!graph) {
+        if (!graph && b->timed) {
             float ms = 0;
             if (b->first_count[bin] > 0 &&
                 hipEventElapsedTime(&ms, ctx->event(EV_FIRST + 2 * bin), ctx->event(EV_FIRST + 2 * bin + 1)) == hipSuccess)
@@ -843,7 +845,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
                 b->trace_ms[bin] = ms;
         }
     }
-    if (!graph && trace) {
+    if (!graph && trace && b->timed) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ctx->event(EV_WALK), ctx->event(EV_WALK + 1));
         b->walk_ms = ms;
@@ -1592,3 +1594,42 @@ int64_t dinara_alignments(int64_t pairs, const char *const *references, const in
 }
 
 }  // extern "C"
+
+namespace dinara {
+
+// dinara_gpu_distances under unit costs with every device buffer taken from `carve` (see carved.h).
+int64_t carved_unit_distances(Carve *carve, int64_t gpu, int64_t pairs, const char *const *references,
+                              const int64_t *reference_lengths, const char *const *queries,
+                              const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                              const dinara_options *options, int64_t threads, int64_t *results) {
+    ParsedCosts parsed = parse_costs(costs);
+    if (parsed.code) return parsed.code;
+    if (const int64_t code = check_mode(mode)) return code;
+    if (parsed.two_piece) return DINARA_INVALID_MODE;
+    const Options o = parse_options(options);
+    const bool unit = parsed.opening == 0 && parsed.del_opening == 0 && parsed.extension == parsed.mismatch &&
+                      parsed.del_extension == parsed.mismatch && o.band_low <= -UNBOUNDED && o.band_high >= UNBOUNDED;
+    if (!unit) return DINARA_INVALID_MODE;
+    if (pairs <= 0) return 0;
+    CarveScope scope(carve);
+    int64_t status = 0;
+    dinara_batch *b = batch_new(gpu, DINARA_BATCH_DISTANCES, pairs, references, reference_lengths, queries,
+                                query_lengths, costs, mode, options, threads, &status);
+    if (!b) return carve->exhausted ? DINARA_SCRATCH_FULL : status;
+    int64_t code = 0;
+    if (!b->unit) {
+        code = DINARA_INVALID_MODE;  // the bit-parallel path switched off
+    } else {
+        // One stream, launches one by one: no graph to instantiate, no side streams or timing events to make.
+        b->use_graph = false;
+        b->side_by_side_bins = false;
+        b->timed = false;
+        code = run(b);
+        if (code == 0) code = dinara_batch_distances(b, results);
+        if (code != 0 && carve->exhausted) code = DINARA_SCRATCH_FULL;
+    }
+    dinara_batch_free(b);
+    return code;
+}
+
+}  // namespace dinara

@@ -38,10 +38,28 @@ struct DeviceGuard {
     ~DeviceGuard();
 };
 
+// A fixed device region handed out front to back, 256-byte aligned: the memory of calls that may not allocate.
+struct Carve {
+    uint8_t *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    bool exhausted = false;    // a reservation found no room
+    void *take(size_t bytes);  // null (and `exhausted` set) when the region has no room left
+};
+
+// While one of these lives on a thread, every DeviceBuffer that thread reserves comes from `carve` and no device
+// memory is allocated; such a buffer is never freed (the region outlives it).
+struct CarveScope {
+    Carve *previous;
+    explicit CarveScope(Carve *carve);
+    ~CarveScope();
+};
+
 // Device memory that grows on demand and is freed with its owner.
 struct DeviceBuffer {
     void *ptr = nullptr;
     size_t size = 0;
+    bool carved = false;  // taken from a Carve: not ours to free
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer &) = delete;
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;

@@ -84,22 +84,45 @@ DeviceGuard::~DeviceGuard() {
     if (previous >= 0) hipSetDevice(previous);
 }
 
+namespace {
+thread_local Carve *active_carve = nullptr;
+}  // namespace
+
+void *Carve::take(size_t bytes) {
+    const size_t at = (used + 255) & ~size_t(255);
+    if (at > size || bytes > size - at) {
+        exhausted = true;
+        return nullptr;
+    }
+    used = at + bytes;
+    return base + at;
+}
+
+CarveScope::CarveScope(Carve *carve) : previous(active_carve) { active_carve = carve; }
+
+CarveScope::~CarveScope() { active_carve = previous; }
+
 DeviceBuffer::~DeviceBuffer() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !carved) (void)hipFree(ptr);
 }
 
 void DeviceBuffer::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !carved) (void)hipFree(ptr);
     ptr = nullptr;
     size = 0;
+    carved = false;
 }
 
 bool DeviceBuffer::reserve(size_t bytes) {
     if (bytes <= size && ptr) return true;
-    if (ptr) {
-        (void)hipFree(ptr);
-        ptr = nullptr;
-        size = 0;
+    release();
+    if (Carve *carve = active_carve) {
+        // A call that may not allocate: the bytes come from its region, or the reservation fails.
+        ptr = carve->take(bytes);
+        if (!ptr) return false;
+        carved = true;
+        size = bytes;
+        return true;
     }
     if (hipMalloc(&ptr, bytes) != hipSuccess) {
         ptr = nullptr;

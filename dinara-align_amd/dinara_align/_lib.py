@@ -19,6 +19,7 @@ OUTSIDE_BAND = -5
 INVALID_MODE = -6
 INVALID_LENGTH = -7
 NO_DEVICE = -8
+SCRATCH_FULL = -9
 
 MESSAGES = {
     UNSUPPORTED_SYMBOLS: "a sequence holds a byte the library refuses",
@@ -29,6 +30,7 @@ MESSAGES = {
     INVALID_MODE: "a mode this library does not serve",
     INVALID_LENGTH: "a sequence's length below zero",
     NO_DEVICE: "no such GPU",
+    SCRATCH_FULL: "the batch needs more than the fixed device scratch",
 }
 
 BATCH_DISTANCES = 0
@@ -83,6 +85,8 @@ def lib() -> ctypes.CDLL:
     handle.dinara_free.restype = None
     handle.dinara_gpu_distances.argtypes = [i64, *batch_args, vp, vp, vp, i64, pi64]
     handle.dinara_gpu_distances.restype = i64
+    handle.dinara_gpu_distances_static.argtypes = [i64, *batch_args, vp, vp, vp, i64, pi64]
+    handle.dinara_gpu_distances_static.restype = i64
     handle.dinara_gpu_alignments.argtypes = [i64, *batch_args, vp, vp, vp, i64, vp, pi64]
     handle.dinara_gpu_alignments.restype = i64
     handle.dinara_distances.argtypes = [*batch_args, vp, vp, vp, i64, pi64]

@@ -33,6 +33,7 @@ extern "C" {
 #define DINARA_INVALID_MODE (-6)        /* a mode this library does not serve */
 #define DINARA_INVALID_LENGTH (-7)      /* a sequence's length below zero */
 #define DINARA_NO_DEVICE (-8)           /* new: no usable GPU, or the GPU id is out of range */
+#define DINARA_SCRATCH_FULL (-9)        /* new: the batch needs more than dinara_gpu_distances_static's scratch */
 
 /* Reference c/dinara.h:59-69. A null pointer is unit costs {1, 0, 1, -1, 0}. */
 typedef struct {
@@ -122,6 +123,24 @@ int64_t dinara_gpu_alignments(int64_t gpu, int64_t pairs, const char *const *ref
                               const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
                               const dinara_options *options, int64_t threads, dinara_alignment *alignments,
                               int64_t *statuses);
+
+/*
+ * New, with no counterpart in the reference: dinara_gpu_distances for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments and codes,
+ * for unit costs (times a scale), global, with no band; other costs return DINARA_INVALID_MODE. Every device
+ * buffer the batch needs (packed letters, queues, counters, results) comes from one fixed array of
+ * DINARA_STATIC_SCRATCH_BYTES in the library, of which each GPU holds a copy from its first call until the process
+ * ends; a batch needing more returns DINARA_SCRATCH_FULL before anything runs (letters at 2 bits and about 444 bytes
+ * a pair must fit; 8-bit letters or ragged groups may need more, which is refused the same way). The call allocates
+ * no device memory and makes no graph, stream or event (beyond the device's own stream and events, made once by the
+ * library's first call on that GPU). Calls on one GPU take turns on a lock; the array keeps the last call's
+ * contents, and nothing in it is read before the call writes it.
+ */
+#define DINARA_STATIC_SCRATCH_BYTES (64ll << 20)
+int64_t dinara_gpu_distances_static(int64_t gpu, int64_t pairs, const char *const *references,
+                                    const int64_t *reference_lengths, const char *const *queries,
+                                    const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
+                                    const dinara_options *options, int64_t threads, int64_t *results);
 
 /* New: a substitution table, as the reference's Mojo/Python `Scoring` (reference scoring.mojo:126-180,
  * python/dinara_align/__init__.py:272-309): `letters` letters of `alphabet`, `table[a * letters + b]` what
