@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <functional>
 #include <atomic>
+#include <memory>
+#include <new>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -35,6 +37,8 @@ struct ParsedCosts {
     int64_t code = 0;
     int64_t mismatch = 1, opening = 0, extension = 1, del_opening = 0, del_extension = 1;
     bool two_piece = false;
+    // The second pieces when two_piece (reference modes.mojo:99-143): a side given none counts its one piece twice.
+    int64_t opening2 = 0, extension2 = 1, del_opening2 = 0, del_extension2 = 1;
 };
 
 ParsedCosts parse_costs(const dinara_costs *c) {
@@ -55,7 +59,11 @@ ParsedCosts parse_costs(const dinara_costs *c) {
         if (c->extension2 <= 0) return invalid();
         if (c->opening2 > MAX_COST || c->extension2 > MAX_COST) return invalid();
         p.two_piece = true;
+        p.opening2 = c->opening2;
+        p.extension2 = c->extension2;
     }
+    p.del_opening2 = p.opening2;
+    p.del_extension2 = p.extension2;
     if (c->deletion_extension != 0) {
         if (c->deletion_extension <= 0 || c->deletion_opening < 0) return invalid();
         if (c->deletion_opening2 >= 0 && c->deletion_extension2 <= 0) return invalid();
@@ -63,7 +71,18 @@ ParsedCosts parse_costs(const dinara_costs *c) {
         if (c->deletion_opening2 > MAX_COST || c->deletion_extension2 > MAX_COST) return invalid();
         p.del_opening = c->deletion_opening;
         p.del_extension = c->deletion_extension;
-        if (c->deletion_opening2 >= 0) p.two_piece = true;
+        if (c->deletion_opening2 >= 0) {
+            p.del_opening2 = c->deletion_opening2;
+            p.del_extension2 = c->deletion_extension2;
+            if (!p.two_piece) {  // insertions gain a second piece: their own once more
+                p.opening2 = p.opening;
+                p.extension2 = p.extension;
+            }
+            p.two_piece = true;
+        } else {
+            p.del_opening2 = p.del_opening;  // deletions lack one: their own once more
+            p.del_extension2 = p.del_extension;
+        }
     }
     return p;
 }
@@ -112,9 +131,9 @@ int64_t pair_code(const char *ref, int64_t n, const char *qry, int64_t m) {
     if (n < 0 || m < 0) return DINARA_INVALID_LENGTH;
     auto plain = [](const char *s, int64_t len) {
         const unsigned char *u = reinterpret_cast<const unsigned char *>(s);
-        for (int64_t i = 0; i < len; ++i)
-            if (u[i] >= 0xFE) return false;
-        return true;
+        unsigned refused = 0;  // no early exit: the loop vectorizes
+        for (int64_t i = 0; i < len; ++i) refused |= u[i] >= 0xFE;
+        return refused == 0;
     };
     if ((n > 0 && !plain(ref, n)) || (m > 0 && !plain(qry, m))) return DINARA_UNSUPPORTED_SYMBOLS;
     return 0;
@@ -249,19 +268,29 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         b->rows[s] = (int32_t)b->reference_length[order[s]];
         b->cols[s] = (int32_t)b->query_length[order[s]];
     }
-    // Two bits a letter when every pair holds at most four distinct bytes (each pair its own codes: only
-    // equality counts), else a byte a letter.
+    // One pass over every pair's letters: which of A, C, G, T it holds and whether it holds any other byte. Two
+    // bits a letter when every pair holds at most four distinct bytes (each pair its own codes: only equality
+    // counts), else a byte a letter.
+    static const ByteClasses classes;
+    std::vector<uint8_t> plain(slots, 0);  // 1: the pair's letters are A, C, G and T alone
     std::atomic<bool> wide{false};
     parallel_for(slots, b->threads, [&](int64_t lo, int64_t hi) {
-        for (int64_t s = lo; s < hi && !wide.load(std::memory_order_relaxed); ++s) {
+        for (int64_t s = lo; s < hi; ++s) {
             const int32_t p = order[s];
-            bool seen[256] = {false};
+            const uint32_t seen = classes.seen(refs[p], b->reference_length[p]) |
+                                  classes.seen(qrys[p], b->query_length[p]);
+            if (!(seen & ByteClasses::OTHER)) {
+                plain[s] = 1;
+                continue;
+            }
+            if (wide.load(std::memory_order_relaxed)) continue;
+            bool present[256] = {false};
             int distinct = 0;
             auto scan = [&](const char *text, int64_t len) {
                 const unsigned char *u = reinterpret_cast<const unsigned char *>(text);
                 for (int64_t i = 0; i < len && distinct <= 4; ++i)
-                    if (!seen[u[i]]) {
-                        seen[u[i]] = true;
+                    if (!present[u[i]]) {
+                        present[u[i]] = true;
                         ++distinct;
                     }
             };
@@ -287,19 +316,36 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         gqry[g] = qw;
         words += (int64_t)(rw + qw) * 64;
     }
-    std::vector<uint32_t> tape(std::max<int64_t>(words, 1), 0);
+    // Every word is written below (a pair's words past its letters are zero), so the tape needs no clearing.
+    std::unique_ptr<uint32_t[]> tape(new (std::nothrow) uint32_t[std::max<int64_t>(words, 1)]);
+    if (!tape) return DINARA_OUT_OF_MEMORY;
+    tape[0] = 0;
     // Ties.LEFT is the rule over both sequences reversed, so their tape is reversed (lanes.mojo:1523-1525).
     const bool reversed = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
     b->reversed = reversed;
     parallel_for(groups, b->threads, [&](int64_t lo, int64_t hi) {
         for (int64_t g = lo; g < hi; ++g) {
-            for (int64_t s = g * 64; s < std::min(slots, (g + 1) * 64); ++s) {
+            const int64_t first = g * 64, last = std::min(slots, (g + 1) * 64);
+            for (int64_t s = first; s < (g + 1) * 64; ++s) {
+                uint32_t *ref_out = tape.get() + gbase[g] + (s & 63);
+                uint32_t *qry_out = ref_out + (int64_t)gref[g] * 64;
+                if (s >= last) {  // the group's empty slots
+                    for (int32_t w = 0; w < gref[g]; ++w) ref_out[(int64_t)w * 64] = 0;
+                    for (int32_t w = 0; w < gqry[g]; ++w) qry_out[(int64_t)w * 64] = 0;
+                    continue;
+                }
                 const int32_t p = order[s];
-                // Two-bit codes: A, C, G and T keep 0, 1, 2, 3 (so a code names the same base in every pair, as
-                // the k-mer features read them), any other byte the least code its pair leaves free. Only
-                // equality counts to the sweeps.
+                const int64_t n = b->reference_length[p], m = b->query_length[p];
+                if (b->bits == 2 && plain[s]) {
+                    // A, C, G and T are codes 0, 1, 2, 3 in every pair (the k-mer features read them so).
+                    classes.pack2(refs[p], n, reversed, ref_out, gref[g]);
+                    classes.pack2(qrys[p], m, reversed, qry_out, gqry[g]);
+                    continue;
+                }
+                // Two-bit codes for a pair with other bytes: A, C, G and T keep 0, 1, 2, 3, any other byte the
+                // least code its pair leaves free; or a byte a letter. Only equality counts to the sweeps.
                 int16_t codes[256];
-                for (int i = 0; i < 256; ++i) codes[i] = -1;
+                for (int i = 0; i < 256; ++i) codes[i] = b->bits == 8 ? (int16_t)i : -1;
                 if (b->bits == 2) {
                     bool used[4] = {false, false, false, false};
                     auto fixed = [](unsigned char byte) { return byte == 'A' ? 0 : byte == 'C' ? 1 : byte == 'G' ? 2 : byte == 'T' ? 3 : -1; };
@@ -319,26 +365,26 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
                             }
                         }
                     };
-                    note(refs[p], b->reference_length[p], true);
-                    note(qrys[p], b->query_length[p], true);
-                    note(refs[p], b->reference_length[p], false);
-                    note(qrys[p], b->query_length[p], false);
+                    note(refs[p], n, true);
+                    note(qrys[p], m, true);
+                    note(refs[p], n, false);
+                    note(qrys[p], m, false);
                 }
-                auto code_of = [&](unsigned char byte) -> uint32_t {
-                    if (b->bits == 8) return byte;
-                    return (uint32_t)codes[byte];
-                };
-                uint32_t *ref_out = tape.data() + gbase[g] + (s & 63);
-                uint32_t *qry_out = ref_out + (int64_t)gref[g] * 64;
-                auto pack = [&](const char *text, int64_t len, uint32_t *out) {
+                auto pack = [&](const char *text, int64_t len, uint32_t *out, int32_t words_here) {
                     const unsigned char *u = reinterpret_cast<const unsigned char *>(text);
-                    for (int64_t i = 0; i < len; ++i) {
-                        const unsigned char byte = reversed ? u[len - 1 - i] : u[i];
-                        out[(i / per) * 64] |= code_of(byte) << (b->bits * (i % per));
+                    for (int32_t w = 0; w < words_here; ++w) {
+                        uint32_t word = 0;
+                        for (int i = 0; i < per; ++i) {
+                            const int64_t at = (int64_t)w * per + i;
+                            if (at >= len) break;
+                            const unsigned char byte = reversed ? u[len - 1 - at] : u[at];
+                            word |= (uint32_t)codes[byte] << (b->bits * i);
+                        }
+                        out[(int64_t)w * 64] = word;
                     }
                 };
-                pack(refs[p], b->reference_length[p], ref_out);
-                pack(qrys[p], b->query_length[p], qry_out);
+                pack(refs[p], n, ref_out, gref[g]);
+                pack(qrys[p], m, qry_out, gqry[g]);
             }
         }
     });
@@ -358,10 +404,11 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     b->group_letters = (b->group_max_letters + 128 + 16 + 3) & ~3;
     // The group sweep holds costs times 16 with a tag (group.hip) and runs up to half its widest band past the
     // corner: every value it makes stays under 2^30.
-    b->groups_fit = group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
+    // The group and block sweeps hold one gap piece: two-piece batches take the wave sweeps alone.
+    b->groups_fit = !b->parsed.two_piece && group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
                     ((int64_t)b->group_max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
     b->block_mask = 0;
-    for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES; ++cls)
+    for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES && !b->parsed.two_piece; ++cls)
         if (block_letters(b->max_letters, cls) <= b->lds_limit &&
             ((int64_t)b->max_letters + class_width(cls) + 8) * b->dearest < REAL_LIMIT &&
             !(block_s(cls) > 8 && b->kind == DINARA_BATCH_ALIGNMENTS))  // the widest keep no decisions
@@ -408,7 +455,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
 
     // Upload.
     DeviceContext *ctx = b->ctx;
-    if (!b->tape.upload(ctx, tape.data(), tape.size() * 4) ||
+    if (!b->tape.upload(ctx, tape.get(), (size_t)std::max<int64_t>(words, 1) * 4) ||
         !b->group_base.upload(ctx, gbase.data(), gbase.size() * 8) ||
         !b->group_ref_words.upload(ctx, gref.data(), gref.size() * 4) ||
         !b->rows_d.upload(ctx, b->rows.data(), std::max<int64_t>(slots, 1) * 4) ||
@@ -601,7 +648,7 @@ EditArgs edit_args(dinara_batch *b) {
     EditArgs e{};
     e.s = sweep_args(b);
     // The certificate and the cap in unit terms: every cost is `unit_scale` times the unit cost.
-    e.s.costs = CostParams{1, 0, 1, 0, 1, 1, 1};
+    e.s.costs = CostParams{1, 0, 1, 0, 1, 1, 1, 1, 0, 1, 0, 1, 1, 1};
     e.s.cap = b->options.max_cost >= 0 ? b->options.max_cost / b->unit_scale : INT64_MAX / 4;
     e.scale = b->unit_scale;
     e.capacity = (int)std::max<size_t>(b->slot_pair.size(), 1);
@@ -980,7 +1027,6 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     ParsedCosts parsed = parse_costs(costs);
     if (parsed.code) return fail(parsed.code);
     if (const int64_t code = check_mode(mode)) return fail(code);
-    if (parsed.two_piece) return fail(DINARA_INVALID_MODE);
     if (kind != DINARA_BATCH_DISTANCES && kind != DINARA_BATCH_ALIGNMENTS) return fail(DINARA_INVALID_MODE);
     DeviceContext *ctx = device_context(gpu);
     if (!ctx) return fail(DINARA_NO_DEVICE);
@@ -1002,6 +1048,13 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     c.ins_ext = (int)parsed.extension;
     c.del_first = c.del_open + c.del_ext;
     c.ins_first = c.ins_open + c.ins_ext;
+    c.pieces = parsed.two_piece ? 2 : 1;
+    c.del_open2 = (int)parsed.del_opening2;
+    c.del_ext2 = (int)parsed.del_extension2;
+    c.ins_open2 = (int)parsed.opening2;
+    c.ins_ext2 = (int)parsed.extension2;
+    c.del_first2 = c.del_open2 + c.del_ext2;
+    c.ins_first2 = c.ins_open2 + c.ins_ext2;
     // The lanes' diagonals are the query's letters less the reference's: the library band mirrored
     // (lanes.mojo:1151-1155).
     const Options &o = b->options;
@@ -1011,12 +1064,14 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     b->user_high = (int)std::min<int64_t>(-o.band_low, clamp);
     // Unit costs times a scale, with no user band, take the bit-parallel path (DINARA_GOTOH_UNIT=1 keeps them on the
     // gap-cost sweeps, for comparison).
-    b->unit = !b->user_band && parsed.opening == 0 && parsed.del_opening == 0 &&
+    b->unit = !b->user_band && !parsed.two_piece && parsed.opening == 0 && parsed.del_opening == 0 &&
               parsed.extension == parsed.mismatch && parsed.del_extension == parsed.mismatch &&
               !(std::getenv("DINARA_GOTOH_UNIT") && std::atoi(std::getenv("DINARA_GOTOH_UNIT")) != 0);
     b->unit_scale = (int)parsed.mismatch;
-    const int64_t dearest = std::max<int64_t>(
+    int64_t dearest = std::max<int64_t>(
         parsed.mismatch, std::max(parsed.opening + parsed.extension, parsed.del_opening + parsed.del_extension));
+    if (parsed.two_piece)
+        dearest = std::max(dearest, std::max(parsed.opening2 + parsed.extension2, parsed.del_opening2 + parsed.del_extension2));
     b->dearest = std::max<int64_t>(dearest, 1);
     b->code.assign(b->pairs, 0);
     b->reference_length.assign(b->pairs, 0);
@@ -1177,8 +1232,16 @@ int64_t seed_slots(dinara_batch *b, GuessOf guess_of) {
     if (slots == 0) return 0;
     // The lanes' off-band price (lanes.mojo:333-345) in the batch's costs, as the device's off_band.
     const CostParams &c = b->costs;
-    auto ins = [&](int64_t k) { return k <= 0 ? 0 : (int64_t)c.ins_open + k * (int64_t)c.ins_ext; };
-    auto del = [&](int64_t k) { return k <= 0 ? 0 : (int64_t)c.del_open + k * (int64_t)c.del_ext; };
+    auto ins = [&](int64_t k) {
+        if (k <= 0) return int64_t(0);
+        const int64_t one = (int64_t)c.ins_open + k * (int64_t)c.ins_ext;
+        return c.pieces == 2 ? std::min(one, (int64_t)c.ins_open2 + k * (int64_t)c.ins_ext2) : one;
+    };
+    auto del = [&](int64_t k) {
+        if (k <= 0) return int64_t(0);
+        const int64_t one = (int64_t)c.del_open + k * (int64_t)c.del_ext;
+        return c.pieces == 2 ? std::min(one, (int64_t)c.del_open2 + k * (int64_t)c.del_ext2) : one;
+    };
     auto off = [&](int64_t k, int64_t end) {
         if (k > std::max<int64_t>(end, 0)) return ins(k) + del(k - end);
         if (k < std::min<int64_t>(end, 0)) return del(-k) + ins(end - k);
