@@ -11,12 +11,21 @@ __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
 // ------------------------------------------------------------------ the band certificate
 
+// A gap's cost at its cheaper piece, nothing for none (reference lanes.mojo:268-286).
 __device__ __forceinline__ int64_t deleted_cost(const CostParams &c, int64_t letters) {
-    return letters <= 0 ? 0 : (int64_t)c.del_open + letters * (int64_t)c.del_ext;
+    if (letters <= 0) return 0;
+    const int64_t one = (int64_t)c.del_open + letters * (int64_t)c.del_ext;
+    if (c.pieces != 2) return one;
+    const int64_t two = (int64_t)c.del_open2 + letters * (int64_t)c.del_ext2;
+    return two < one ? two : one;
 }
 
 __device__ __forceinline__ int64_t inserted_cost(const CostParams &c, int64_t letters) {
-    return letters <= 0 ? 0 : (int64_t)c.ins_open + letters * (int64_t)c.ins_ext;
+    if (letters <= 0) return 0;
+    const int64_t one = (int64_t)c.ins_open + letters * (int64_t)c.ins_ext;
+    if (c.pieces != 2) return one;
+    const int64_t two = (int64_t)c.ins_open2 + letters * (int64_t)c.ins_ext2;
+    return two < one ? two : one;
 }
 
 // The least any path visiting lane diagonal k (query letters less reference letters) costs, its end on lane
@@ -124,6 +133,42 @@ __device__ __forceinline__ int64_t walk_rows(const SweepArgs &a, int slot, int n
         }
         const bool extended = layer == 1 ? ((f >> 2) & 1) : ((f >> 3) & 1);
         if (layer == 1) {
+            ops.put(2u);
+            --r;
+        } else {
+            ops.put(3u);
+            --c;
+        }
+        if (!extended) layer = 0;
+    }
+    for (; r > 0; --r) ops.put(2u);
+    for (; c > 0; --c) ops.put(3u);
+    ops.close();
+    return ops.count;
+}
+
+// The same walk over two-piece decisions, a byte a cell: bits 0-2 the source the aligned layer takes (0 two equal
+// letters, 5 two unequal, 1 the first piece's deletions, 2 its insertions, 3 the second piece's deletions, 4 its
+// insertions), bit 2 + L gap layer L (1..4) extends into the cell. Odd layers delete, even ones insert.
+template <typename FlagAt>
+__device__ __forceinline__ int64_t walk_rows2(const SweepArgs &a, int slot, int n, int m, int lo, FlagAt flag_at) {
+    OpTape ops = op_tape(a, slot);
+    const uint32_t unequal = a.eqx ? 1u : 0u;
+    int r = n, c = m, layer = 0;
+    while (r > 0 && c > 0) {
+        const uint32_t f = flag_at(r, c - r - lo);
+        if (layer == 0) {
+            const uint32_t src = f & 7;
+            if (src == 0 || src == 5) {
+                ops.put(src == 0 ? 0u : unequal);
+                --r;
+                --c;
+                continue;
+            }
+            layer = (int)src;
+        }
+        const bool extended = (f >> (2 + layer)) & 1;
+        if (layer & 1) {
             ops.put(2u);
             --r;
         } else {

@@ -17,6 +17,11 @@ struct CostParams {
     int mismatch;
     int del_open, del_ext, ins_open, ins_ext;
     int del_first, ins_first;  // a gap's first letter: opening plus extension
+    // Two-piece costs (reference lanes.mojo:187-345): a gap costs the less of its two pieces; with one piece
+    // (pieces == 1) the second's fields are unused.
+    int pieces;
+    int del_open2, del_ext2, ins_open2, ins_ext2;
+    int del_first2, ins_first2;
 };
 
 // Which pairs a launch takes and where their letters lie. The tape holds every sequence 2 or 8 bits a

@@ -23,7 +23,13 @@ namespace dinara {
 
 // ------------------------------------------------------------------ one wave per pair
 
-template <int Q, bool RECORD>
+// P = 2: two gap pieces (reference lanes.mojo:795-947 with pieces == 2). Each piece's deletion layer comes from the
+// row above as the first's does; each piece's insertion run is a prefix minimum of its own along the row. A run
+// mixing the two pieces never costs less than the better piece alone (its openings include the cheaper-extending
+// piece's, its letters cost no less), so the cells' values, and the decisions on every path the walk can take,
+// are the recurrence's. Decisions are then a byte a cell: bits 0-2 the source (0 two equal letters, 5 two unequal,
+// 1 and 2 the first piece's deletion and insertion layers, 3 and 4 the second's), bit 2 + L layer L extends.
+template <int Q, bool RECORD, int P>
 __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
     extern __shared__ uint8_t letters[];
     const int t = blockIdx.x;
@@ -51,12 +57,15 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
     __syncthreads();
 
     const int first = lo + lane * Q;  // this lane's lowest diagonal
-    int H[Q], D[Q];
+    int H[Q], D[Q], D2[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int kk = first + i;
-        H[i] = kk < 0 ? INF : (kk == 0 ? 0 : (kk <= m ? k.ins_open + kk * k.ins_ext : INF));
+        int top = kk <= m ? k.ins_open + kk * k.ins_ext : INF;
+        if (P == 2 && kk <= m) top = imin(top, k.ins_open2 + kk * k.ins_ext2);
+        H[i] = kk < 0 ? INF : (kk == 0 ? 0 : top);
         D[i] = INF;
+        D2[i] = INF;
     }
     const bool barred_band = a.user_band != 0;
     const int user_low = a.user_low, user_high = a.user_high;
@@ -68,12 +77,14 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
         // The row above's next diagonal, for the lane's last one: the next lane's first.
         int hu_next = __shfl_down(H[0], 1);
         int du_next = __shfl_down(D[0], 1);
-        if (lane == 63) hu_next = du_next = INF;
-        int T[Q], SUB[Q], DN[Q], X[Q];
+        int du2_next = P == 2 ? __shfl_down(D2[0], 1) : INF;
+        if (lane == 63) hu_next = du_next = du2_next = INF;
+        int T[Q], SUB[Q], DN[Q], DN2[Q], X[Q], X2[Q];
         bool EQ[Q];
-        bool ED_LE[Q];
-        const int border = k.del_open + r * k.del_ext;
-        int aggregate = INF;
+        bool ED_LE[Q], ED2_LE[Q];
+        int border = k.del_open + r * k.del_ext;
+        if (P == 2) border = imin(border, k.del_open2 + r * k.del_ext2);
+        int aggregate = INF, aggregate2 = INF;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int kk = first + i;
@@ -82,32 +93,52 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
             const int du = i + 1 < Q ? D[i + 1] : du_next;
             const int od = hu + k.del_first, ed = du + k.del_ext;
             const int dn = imin(od, ed);
+            int dn2 = INF;
+            bool ed2_le = false;
+            if (P == 2) {
+                const int du2 = i + 1 < Q ? D2[i + 1] : du2_next;
+                const int od2 = hu + k.del_first2, ed2 = du2 + k.del_ext2;
+                dn2 = imin(od2, ed2);
+                ed2_le = ed2 <= od2;
+            }
             const uint8_t b = (c >= 1 && c <= m) ? letters[n + c - 1] : (uint8_t)0xFF;
             const bool eq = b == letter;
             const int sub = H[i] + (eq ? 0 : k.mismatch);
-            int tt = imin(sub, dn);
+            int tt = imin(sub, imin(dn, dn2));
+            const bool barred = barred_band && (kk < user_low || kk > user_high);
             if (c <= 0) tt = c == 0 ? border : INF;
-            if (barred_band && (kk < user_low || kk > user_high)) tt = INF;
+            if (barred) tt = INF;
             T[i] = tt;
             SUB[i] = sub;
-            DN[i] = (c <= 0 || (barred_band && (kk < user_low || kk > user_high))) ? INF : dn;
+            DN[i] = (c <= 0 || barred) ? INF : dn;
+            DN2[i] = (c <= 0 || barred) ? INF : dn2;
             EQ[i] = eq;
             ED_LE[i] = ed <= od;
+            ED2_LE[i] = ed2_le;
             // The insertion run from diagonal kk onward: its cost less kk extensions, so the run's
             // extensions cancel and the row's insertions are a prefix minimum (the row's own index drops out).
             X[i] = tt - kk * k.ins_ext;
             aggregate = imin(aggregate, X[i]);
+            if (P == 2) {
+                X2[i] = tt - kk * k.ins_ext2;
+                aggregate2 = imin(aggregate2, X2[i]);
+            }
         }
         // Inclusive minimum across the wave, then shifted by one lane: the best run start below this lane.
-        int scan = aggregate;
+        int scan = aggregate, scan2 = aggregate2;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int other = __shfl_up(scan, d);
             if (lane >= d) scan = imin(scan, other);
+            if (P == 2) {
+                const int other2 = __shfl_up(scan2, d);
+                if (lane >= d) scan2 = imin(scan2, other2);
+            }
         }
         int before = __shfl_up(scan, 1);
-        if (lane == 0) before = INF + INF / 2;
-        int I[Q], HN[Q];
+        int before2 = P == 2 ? __shfl_up(scan2, 1) : INF;
+        if (lane == 0) before = before2 = INF + INF / 2;
+        int I[Q], I2[Q], HN[Q];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const int kk = first + i;
@@ -116,16 +147,26 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
             ii = ii < INF ? ii : INF;
             before = imin(before, X[i]);
             const int c = r + kk;
-            if (c <= 0 || (barred_band && (kk < user_low || kk > user_high))) ii = INF;
+            const bool off = c <= 0 || (barred_band && (kk < user_low || kk > user_high));
+            if (off) ii = INF;
             I[i] = ii;
             int h = imin(T[i], ii);
+            if (P == 2) {
+                int i2 = before2 + (k.ins_first2 - k.ins_ext2) + kk * k.ins_ext2;
+                i2 = i2 < INF ? i2 : INF;
+                before2 = imin(before2, X2[i]);
+                if (off) i2 = INF;
+                I2[i] = i2;
+                h = imin(h, i2);
+            }
             if (barred_band && (kk < user_low || kk > user_high)) h = INF;
             HN[i] = h;
         }
         if (RECORD) {
             int hl_prev = __shfl_up(HN[Q - 1], 1);
             int il_prev = __shfl_up(I[Q - 1], 1);
-            if (lane == 0) hl_prev = il_prev = INF;
+            int il2_prev = P == 2 ? __shfl_up(I2[Q - 1], 1) : INF;
+            if (lane == 0) hl_prev = il_prev = il2_prev = INF;
             uint32_t packed = 0;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
@@ -133,11 +174,26 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
                 const int il = i > 0 ? I[i - 1] : il_prev;
                 const int oi = hl + k.ins_first, ei = il + k.ins_ext;
                 const int h = HN[i];
-                uint32_t src = EQ[i] ? 0u : 3u;
-                src = I[i] == h ? 2u : src;
-                src = DN[i] == h ? 1u : src;
-                src = (!EQ[i] && SUB[i] == h) ? 3u : src;
-                const uint32_t flag = src | (ED_LE[i] ? 4u : 0u) | (ei <= oi ? 8u : 0u);
+                uint32_t flag;
+                if (P == 1) {
+                    // The lowest priority first, each later source as cheap as the cell taking over.
+                    uint32_t src = EQ[i] ? 0u : 3u;
+                    src = I[i] == h ? 2u : src;
+                    src = DN[i] == h ? 1u : src;
+                    src = (!EQ[i] && SUB[i] == h) ? 3u : src;
+                    flag = src | (ED_LE[i] ? 4u : 0u) | (ei <= oi ? 8u : 0u);
+                } else {
+                    const int il2 = i > 0 ? I2[i - 1] : il2_prev;
+                    const int oi2 = hl + k.ins_first2, ei2 = il2 + k.ins_ext2;
+                    uint32_t src = EQ[i] ? 0u : 5u;
+                    src = I[i] == h ? 2u : src;
+                    src = I2[i] == h ? 4u : src;
+                    src = DN[i] == h ? 1u : src;
+                    src = DN2[i] == h ? 3u : src;
+                    src = (!EQ[i] && SUB[i] == h) ? 5u : src;
+                    flag = src | (ED_LE[i] ? 8u : 0u) | (ei <= oi ? 16u : 0u) | (ED2_LE[i] ? 32u : 0u) |
+                           (ei2 <= oi2 ? 64u : 0u);
+                }
                 if (Q % 4 == 0) {
                     packed |= flag << (8 * (i % 4));
                     if (i % 4 == 3) {
@@ -153,6 +209,7 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
         for (int i = 0; i < Q; ++i) {
             H[i] = HN[i];
             D[i] = DN[i];
+            if (P == 2) D2[i] = DN2[i];
         }
     }
 
@@ -216,9 +273,10 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
             });
         } else {
             const int W = band_width(cls);
-            moves = walk_rows(a, slot, n, m, lo, [&](int r, int j) -> uint32_t {
+            auto flag_at = [&](int r, int j) -> uint32_t {
                 return (unsigned)j < (unsigned)W ? base[(size_t)(r - 1) * W + j] : 0u;
-            });
+            };
+            moves = a.costs.pieces == 2 ? walk_rows2(a, slot, n, m, lo, flag_at) : walk_rows(a, slot, n, m, lo, flag_at);
         }
     }
     publish_cigars(a, slot, moves);
@@ -229,8 +287,13 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
 template <int Q>
 static void launch_wave(const SweepArgs &a, size_t lds, hipStream_t stream) {
     const dim3 grid(a.count), block(64);
-    if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true>), grid, block, lds, stream, a);
-    else hipLaunchKernelGGL((sweep_wave<Q, false>), grid, block, lds, stream, a);
+    if (a.costs.pieces == 2) {
+        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 2>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_wave<Q, false, 2>), grid, block, lds, stream, a);
+    } else {
+        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 1>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_wave<Q, false, 1>), grid, block, lds, stream, a);
+    }
 }
 
 int class_width(int cls) { return band_width(cls); }
