@@ -863,3 +863,169 @@ int64_t orc_check_alignments(int64_t pairs, const uint8_t *const *refs, const in
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------ free ends */
+
+/* Gotoh's recurrence over the whole matrix with free starts (reference tests/test_alignment.mojo:1944-1980, the
+ * lanes' LaneEnds of lanes.mojo:150-185): a cell (r, 0) with r <= start_rows or (0, c) with c <= start_columns
+ * starts an alignment for nothing, every other cell takes the recurrence (borders from their one neighbour).
+ * Cells off diagonals band_low..band_high (reference letters less query letters) are reached by nothing. */
+static int64_t *ends_matrix(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                            int64_t start_rows, int64_t start_columns, int64_t band_low, int64_t band_high) {
+    int64_t stride = m + 1;
+    size_t cells = (size_t)((n + 1) * stride);
+    int64_t *h = malloc(sizeof(int64_t) * cells);
+    int64_t *d = malloc(sizeof(int64_t) * cells), *in = malloc(sizeof(int64_t) * cells);
+    int64_t *d2 = malloc(sizeof(int64_t) * cells), *in2 = malloc(sizeof(int64_t) * cells);
+    for (int64_t r = 0; r <= n; r++) {
+        for (int64_t c = 0; c <= m; c++) {
+            size_t at = (size_t)(r * stride + c);
+            d[at] = in[at] = d2[at] = in2[at] = INF;
+            if (r - c < band_low || r - c > band_high) {
+                h[at] = INF;
+                continue;
+            }
+            if ((c == 0 && r <= start_rows) || (r == 0 && c <= start_columns)) {
+                h[at] = 0;
+                continue;
+            }
+            int64_t best = INF;
+            if (r > 0) {
+                size_t up = at - stride;
+                d[at] = min64(h[up] + costs->del_open + costs->del_ext, d[up] + costs->del_ext);
+                best = min64(best, d[at]);
+                if (costs->pieces == 2) {
+                    d2[at] = min64(h[up] + costs->del_open2 + costs->del_ext2, d2[up] + costs->del_ext2);
+                    best = min64(best, d2[at]);
+                }
+            }
+            if (c > 0) {
+                size_t left = at - 1;
+                in[at] = min64(h[left] + costs->ins_open + costs->ins_ext, in[left] + costs->ins_ext);
+                best = min64(best, in[at]);
+                if (costs->pieces == 2) {
+                    in2[at] = min64(h[left] + costs->ins_open2 + costs->ins_ext2, in2[left] + costs->ins_ext2);
+                    best = min64(best, in2[at]);
+                }
+            }
+            if (r > 0 && c > 0) best = min64(best, h[at - stride - 1] + (ref[r - 1] == qry[c - 1] ? 0 : costs->mismatch));
+            h[at] = min64(best, INF);
+            d[at] = min64(d[at], INF);
+            in[at] = min64(in[at], INF);
+            d2[at] = min64(d2[at], INF);
+            in2[at] = min64(in2[at], INF);
+        }
+    }
+    free(d);
+    free(in);
+    free(d2);
+    free(in2);
+    return h;
+}
+
+int64_t orc_ends_cost(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                      const int64_t ends[4], int64_t band_low, int64_t band_high, int64_t *end_row,
+                      int64_t *end_column) {
+    /* Of the cells an alignment may end on, the least, ties to the highest diagonal (reference
+     * tests/test_alignment.mojo:1983-2000 rule_span). */
+    int64_t *h = ends_matrix(ref, n, qry, m, costs, ends[0], ends[2], band_low, band_high);
+    int64_t stride = m + 1, least = INF, er = -1, ec = -1;
+    for (int64_t r = 0; r <= n; r++)
+        for (int64_t c = 0; c <= m; c++) {
+            if (!((r == n && m - c <= ends[3]) || (c == m && n - r <= ends[1]))) continue;
+            int64_t value = h[r * stride + c];
+            if (value < least || (value == least && value < INF && r - c > er - ec)) {
+                least = value;
+                er = r;
+                ec = c;
+            }
+        }
+    free(h);
+    if (end_row) *end_row = er;
+    if (end_column) *end_column = ec;
+    return least >= INF ? ORC_OUTSIDE : least;
+}
+
+/* The left tie rule's span and CIGAR (reference tests/test_alignment.mojo:1983-2020, 2094-2110): the end as above;
+ * then, back from that end over both prefixes reversed with no free letters, of the starts an optimum leaves from,
+ * the one on the highest diagonal; the letters between aligned globally by the same rule. */
+static int64_t ends_align_left(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                               const int64_t ends[4], int64_t band_low, int64_t band_high, int eqx, char *cigar,
+                               int64_t cap, int64_t spans[4]) {
+    int64_t er, ec;
+    int64_t least = orc_ends_cost(ref, n, qry, m, costs, ends, band_low, band_high, &er, &ec);
+    if (least == ORC_OUTSIDE) return ORC_OUTSIDE;
+    uint8_t *a = malloc((size_t)er + 1), *b = malloc((size_t)ec + 1);
+    for (int64_t i = 0; i < er; i++) a[i] = ref[er - 1 - i];
+    for (int64_t i = 0; i < ec; i++) b[i] = qry[ec - 1 - i];
+    /* Diagonal k of the reversed prefixes is er - ec - k of the pair. */
+    int64_t low = band_high >= ORC_NO_BAND_HIGH ? ORC_NO_BAND_LOW : (er - ec) - band_high;
+    int64_t high = band_low <= ORC_NO_BAND_LOW ? ORC_NO_BAND_HIGH : (er - ec) - band_low;
+    int64_t *back = ends_matrix(a, er, b, ec, costs, 0, 0, low, high);
+    int64_t sr = -1, sc = -1, stride = ec + 1;
+    for (int64_t r = 0; r <= er; r++)
+        for (int64_t c = 0; c <= ec; c++) {
+            if (!((c == 0 && r <= ends[0]) || (r == 0 && c <= ends[2]))) continue;
+            if (back[(er - r) * stride + (ec - c)] != least) continue;
+            if (sr < 0 || r - c > sr - sc) {
+                sr = r;
+                sc = c;
+            }
+        }
+    free(back);
+    free(a);
+    free(b);
+    if (sr < 0) return -9; /* no start reaches the end at the least cost: never, if the recurrences agree */
+    int64_t inner_low = band_low <= ORC_NO_BAND_LOW ? ORC_NO_BAND_LOW : band_low - (sr - sc);
+    int64_t inner_high = band_high >= ORC_NO_BAND_HIGH ? ORC_NO_BAND_HIGH : band_high - (sr - sc);
+    int64_t cost = orc_costs_align(ref + sr, er - sr, qry + sc, ec - sc, costs, inner_low, inner_high, 0, eqx, cigar,
+                                   cap);
+    spans[0] = sr;
+    spans[1] = er;
+    spans[2] = sc;
+    spans[3] = ec;
+    return cost < 0 ? cost : (cost == least ? least : -10);
+}
+
+static void reverse_runs(char *cigar) {
+    /* A CIGAR's runs in the other order. */
+    size_t length = strlen(cigar);
+    char *copy = malloc(length + 1);
+    size_t out = 0;
+    size_t end = length;
+    while (end > 0) {
+        size_t start = end - 1; /* the run's operation */
+        while (start > 0 && cigar[start - 1] >= '0' && cigar[start - 1] <= '9') start--;
+        memcpy(copy + out, cigar + start, end - start);
+        out += end - start;
+        end = start;
+    }
+    copy[out] = 0;
+    memcpy(cigar, copy, out + 1);
+    free(copy);
+}
+
+int64_t orc_ends_align(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                       const int64_t ends[4], int64_t band_low, int64_t band_high, int right_ties, int eqx, char *cigar,
+                       int64_t cap, int64_t spans[4]) {
+    if (!right_ties) return ends_align_left(ref, n, qry, m, costs, ends, band_low, band_high, eqx, cigar, cap, spans);
+    /* Ties.RIGHT: the left rule over both sequences reversed, the free letters of each end swapped and the band
+     * mirrored about the end diagonal, its span and CIGAR read back (reference tests/test_alignment.mojo:2054-2093). */
+    uint8_t *a = malloc((size_t)n + 1), *b = malloc((size_t)m + 1);
+    for (int64_t i = 0; i < n; i++) a[i] = ref[n - 1 - i];
+    for (int64_t i = 0; i < m; i++) b[i] = qry[m - 1 - i];
+    int64_t flipped[4] = {ends[1], ends[0], ends[3], ends[2]};
+    int64_t low = band_high >= ORC_NO_BAND_HIGH ? ORC_NO_BAND_LOW : (n - m) - band_high;
+    int64_t high = band_low <= ORC_NO_BAND_LOW ? ORC_NO_BAND_HIGH : (n - m) - band_low;
+    int64_t mirrored[4];
+    int64_t cost = ends_align_left(a, n, b, m, costs, flipped, low, high, eqx, cigar, cap, mirrored);
+    free(a);
+    free(b);
+    if (cost < 0) return cost;
+    reverse_runs(cigar);
+    spans[0] = n - mirrored[1];
+    spans[1] = n - mirrored[0];
+    spans[2] = m - mirrored[3];
+    spans[3] = m - mirrored[2];
+    return cost;
+}

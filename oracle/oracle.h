@@ -85,6 +85,22 @@ int64_t orc_off_band(const orc_costs *costs, int64_t diagonal, int64_t end);
 int orc_proof(const orc_costs *costs, int64_t rows, int64_t columns, int64_t found, int64_t low, int64_t high,
               int strict, int64_t *wide_low, int64_t *wide_high);
 
+/* Free ends (reference gap_affine.mojo:421-448 EndsFree, tests/test_alignment.mojo:1944-2020): ends[0] reference
+ * letters an alignment may leave out at the start for nothing, ends[1] at the end, ends[2] and ends[3] the query's.
+ * orc_ends_cost: the least cost over every start and end so allowed, and the end the tie rule takes (of the optimal
+ * ends, the one on the highest diagonal, reference letters less query letters), or ORC_OUTSIDE. */
+int64_t orc_ends_cost(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                      const int64_t ends[4], int64_t band_low, int64_t band_high, int64_t *end_row,
+                      int64_t *end_column);
+
+/* The alignment the tie rule names under free ends: Ties.LEFT takes the end above, then the start on the highest
+ * diagonal an optimum ending there leaves from, and the letters between aligned globally by the left rule;
+ * Ties.RIGHT is that over both sequences reversed with each sequence's free letters swapped end for end, read back.
+ * Returns the cost (or ORC_OUTSIDE), writes the CIGAR and spans = {reference start, end, query start, end}. */
+int64_t orc_ends_align(const uint8_t *ref, int64_t n, const uint8_t *qry, int64_t m, const orc_costs *costs,
+                       const int64_t ends[4], int64_t band_low, int64_t band_high, int right_ties, int eqx, char *cigar,
+                       int64_t cap, int64_t spans[4]);
+
 /* Scoring: a substitution table `table[a * letters + b]` (a the reference's code), a gap of k letters scoring
  * `open + (k - 1) * extend` (both <= 0; the library's Scoring(opening, extension) has open = opening +
  * extension, extend = extension, reference scoring.mojo:202-214). Sequences are codes 0..letters-1. */

@@ -66,6 +66,10 @@ def _load() -> ctypes.CDLL:
     vp = ctypes.c_void_p
     lib.orc_check_alignments.argtypes = [i64, vp, vp, vp, vp, vp, vp, vp, vp, pi64]
     lib.orc_check_alignments.restype = i64
+    lib.orc_ends_cost.argtypes = [p8, i64, p8, i64, vp, vp, i64, i64, pi64, pi64]
+    lib.orc_ends_cost.restype = i64
+    lib.orc_ends_align.argtypes = [p8, i64, p8, i64, vp, vp, i64, i64, ctypes.c_int, ctypes.c_int, vp, i64, vp]
+    lib.orc_ends_align.restype = i64
     del pc
     return lib
 
@@ -325,3 +329,34 @@ def rows_cost(top: str, bottom: str, costs: Costs) -> int:
     if run is not None:
         total += gap(run, length)
     return total
+
+
+def _ends(ends, n, m):
+    """Free letters {reference start, end, query start, end}, none past its sequence (EndsFree.of)."""
+    return (ctypes.c_int64 * 4)(min(ends[0], n), min(ends[1], n), min(ends[2], m), min(ends[3], m))
+
+
+def ends_cost(ref, qry, costs: Costs, ends, band: Tuple[int, int] = (NO_BAND_LOW, NO_BAND_HIGH)):
+    """(cost or OUTSIDE, end row, end column) under free ends `ends` = (reference start, end, query start, end)."""
+    a, b = _b(ref), _b(qry)
+    c = costs._c()
+    er, ec = ctypes.c_int64(0), ctypes.c_int64(0)
+    cost = lib().orc_ends_cost(a, len(a), b, len(b), ctypes.byref(c), _ends(ends, len(a), len(b)), band[0], band[1],
+                               ctypes.byref(er), ctypes.byref(ec))
+    return cost, er.value, ec.value
+
+
+def ends_align(ref, qry, costs: Costs, ends, band: Tuple[int, int] = (NO_BAND_LOW, NO_BAND_HIGH),
+               right_ties: bool = False, eqx: bool = True):
+    """(cost, CIGAR, (reference start, end, query start, end)) the tie rule names under free ends; cost OUTSIDE
+    (and no CIGAR) when no alignment stays in the band."""
+    a, b = _b(ref), _b(qry)
+    c = costs._c()
+    cap = 2 * (len(a) + len(b)) + 2
+    buf = ctypes.create_string_buffer(cap)
+    spans = (ctypes.c_int64 * 4)()
+    cost = lib().orc_ends_align(a, len(a), b, len(b), ctypes.byref(c), _ends(ends, len(a), len(b)), band[0],
+                                band[1], int(right_ties), int(eqx), buf, cap, spans)
+    if cost < 0:
+        return cost, None, None
+    return cost, buf.value.decode("ascii"), tuple(spans)
