@@ -72,6 +72,41 @@ __device__ __forceinline__ bool certify(const SweepArgs &a, int rows, int cols, 
     return false;
 }
 
+// off_band with free letters (reference lanes.mojo:333-345 with LaneEnds, :150-185): a path may start on lane
+// diagonals -start_rows .. start_columns and end on end - end_columns .. end + end_rows.
+__device__ __forceinline__ int64_t off_band_ends(const CostParams &c, int64_t k, int64_t end, int start_rows,
+                                                 int end_rows, int start_columns, int end_columns) {
+    const int64_t start_low = -start_rows, start_high = start_columns;
+    const int64_t end_low = end - end_columns, end_high = end + end_rows;
+    const int64_t high = start_high > end_high ? start_high : end_high;
+    const int64_t low = start_low < end_low ? start_low : end_low;
+    if (k > high) return inserted_cost(c, k - start_high) + deleted_cost(c, k - end_high);
+    if (k < low) return deleted_cost(c, start_low - k) + inserted_cost(c, end_low - k);
+    return 0;
+}
+
+// certify with free letters (no user band): the same proof and the same next band, off_band_ends pricing.
+__device__ __forceinline__ bool certify_ends(const SweepArgs &a, int rows, int cols, int64_t found, int lo, int hi,
+                                             const int (&f)[4], int *want_low, int *want_high) {
+    const int end = cols - rows;
+    const int top = cols, bottom = -rows;
+    const int64_t far = INT64_MAX / 4;
+    auto off = [&](int64_t k) { return off_band_ends(a.costs, k, end, f[0], f[1], f[2], f[3]); };
+    const int64_t beyond_high = hi < top ? off(hi + 1) : far;
+    const int64_t beyond_low = lo > bottom ? off(lo - 1) : far;
+    const int64_t least = beyond_high < beyond_low ? beyond_high : beyond_low;
+    const bool beaten = a.strict ? found < least : found <= least;
+    if (beaten || least > a.cap) return true;
+    const int64_t capped = found < a.cap ? found : a.cap;
+    const int64_t target = a.strict ? capped + 1 : (found <= a.cap ? found : a.cap + 1);
+    int h = hi, l = lo;
+    while (h < top && off(h + 1) < target) ++h;
+    while (l > bottom && off(l - 1) < target) --l;
+    *want_low = l;
+    *want_high = h;
+    return false;
+}
+
 // The class of band a pair of `letters` letters needing `need` diagonals takes next: the narrowest that holds them.
 // A group or block band is placed on an even diagonal, so it holds one diagonal fewer than its width. Pairs under a
 // user band take the wave bands, which bar diagonals outside it; pairs longer than a group launch stages take
@@ -118,7 +153,7 @@ template <typename FlagAt>
 __device__ __forceinline__ int64_t walk_rows(const SweepArgs &a, int slot, int n, int m, int lo, FlagAt flag_at) {
     OpTape ops = op_tape(a, slot);
     const uint32_t unequal = a.eqx ? 1u : 0u;
-    int r = n, c = m, layer = 0;
+    int r = n, c = m, layer = 0;  // the walk's start: the corner, or a free end's cell
     while (r > 0 && c > 0) {
         const uint32_t f = flag_at(r, c - r - lo);
         if (layer == 0) {

@@ -103,9 +103,14 @@ int64_t check_mode(const dinara_mode *m) {
     if (std::min(std::min(m->reference_start, m->reference_end), std::min(m->query_start, m->query_end)) < 0)
         return DINARA_INVALID_MODE;
     if (m->match_score < 0 || m->match_score > MAX_COST) return DINARA_INVALID_COSTS;
-    const bool global = (m->reference_start | m->reference_end | m->query_start | m->query_end) == 0;
-    if (!global || m->match_score > 0) return DINARA_INVALID_MODE;
+    // Free ends that minimize the costs are served; a match score (maximized) is not.
+    if (m->match_score > 0) return DINARA_INVALID_MODE;
     return 0;
+}
+
+// Whether a valid mode frees any letters at an end (reference modes.mojo:261-386: ends_free and its presets).
+bool free_mode(const dinara_mode *m) {
+    return m && (m->reference_start | m->reference_end | m->query_start | m->query_end) != 0;
 }
 
 struct Options {
@@ -192,12 +197,23 @@ struct dinara_batch {
     bool groups_fit = false;              // four pairs' letters fit one block's LDS, their costs the tags
     int block_mask = 0;                   // the block classes whose letters and tags fit
     int64_t dearest = 1;                  // the dearest single step: a substitution or a gap's first letter
+    // Free ends (wave sweeps): the letters each end of each sequence may leave out (reference start, end, query
+    // start, end), the phase a run is in (1 the forward phase finding cost and end, 2 the anchored phase keeping
+    // decisions), and per slot its views, free letters and first bands of each phase.
+    bool free_ends = false;
+    int64_t ends[4] = {0, 0, 0, 0};
+    int phase = 0;
+    std::vector<int32_t> a_rows, a_cols, a_at, a_free, a_first_low, a_first_high, a_first_all;
+    std::vector<int64_t> a_first_offset;
+    std::vector<int32_t> a_status_h, a_cost_h, a_end_row_h, a_end_col_h, b_end_row_h, b_end_col_h;
+    std::vector<int32_t> b_rows_h, b_cols_h;
     std::vector<DeviceBuffer *> arenas;   // one per launch of a run, kept from run to run
     std::vector<int64_t> first_offset;    // where each class's first list starts in first_lists_d
     // device memory
     DeviceBuffer tape, group_base, group_ref_words, rows_d, cols_d, want_low, want_high, first_low_d,
         first_high_d, cost, status, walk_low, walk_class, cigar_scratch, cigar_scratch_base, cigars, cigar_used,
-        cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, walk_flags;
+        cigar_offset, cigar_length, retry_lists, retry_counts, first_lists_d, walk_flags, view_rows_d, view_cols_d,
+        view_ref_at_d, view_qry_at_d, free_d, end_row_d, end_col_d;
     ~dinara_batch() {
         for (DeviceBuffer *buffer : arenas) delete buffer;
         if (edit_graph) (void)hipGraphExecDestroy(edit_graph);
@@ -230,9 +246,11 @@ struct dinara_batch {
     int seeded_start[EDIT_LENGTH_CLASSES] = {-1};  // the start bins the device seeding's launch shape was made for
     DeviceBuffer edit_band_low, edit_first_low_d, edit_lists, edit_retry_lists, edit_retry_counts, edit_later_lists,
         edit_later_counts, edit_counts, edit_first_lists, edit_first_counts, trace_lists,
-        trace_counts, trace_offset, walk_ready, walk_moves, walk_early, deferred, deferred_count, deferred_held, edit_arena,
+        trace_counts, trace_offset, walk_ready, walk_moves, walk_early, deferred, deferred_count, deferred_held,
         arena_used, edit_stats, edit_work;
-    size_t arena_capacity = 0;
+    size_t arena_capacity = 0;            // the bytes of the context's arena this batch's runs use
+    const void *arena_ptr = nullptr;      // where that arena was when the captured graph was made
+    size_t arena_want = 0;                // the bytes it asks for (0: not worked out yet)
     hipGraphExec_t edit_graph = nullptr;
     double edit_ms[EDIT_BINS] = {}, trace_ms[EDIT_BINS] = {}, edit_sweeps[EDIT_BINS] = {},
            trace_sweeps[EDIT_BINS] = {}, graph_used = 0, trace_passes = 0, walked_first = 0;
@@ -316,12 +334,15 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         gqry[g] = qw;
         words += (int64_t)(rw + qw) * 64;
     }
-    // Every word is written below (a pair's words past its letters are zero), so the tape needs no clearing.
-    std::unique_ptr<uint32_t[]> tape(new (std::nothrow) uint32_t[std::max<int64_t>(words, 1)]);
+    // Every word is written below (a pair's words past its letters are zero), so the tape needs no clearing but
+    // its padding: the unit-cost sweeps' letter cursors read up to two words past a sequence's last without a bounds
+    // check (edit.hip), which past the last group's words lands here.
+    const int64_t padded = words + TAPE_PADDING_WORDS;
+    std::unique_ptr<uint32_t[]> tape(new (std::nothrow) uint32_t[padded]);
     if (!tape) return DINARA_OUT_OF_MEMORY;
-    tape[0] = 0;
+    std::fill(tape.get() + words, tape.get() + padded, 0u);
     // Ties.LEFT is the rule over both sequences reversed, so their tape is reversed (lanes.mojo:1523-1525).
-    const bool reversed = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties;
+    const bool reversed = b->kind == DINARA_BATCH_ALIGNMENTS && !b->options.right_ties && !b->free_ends;
     b->reversed = reversed;
     parallel_for(groups, b->threads, [&](int64_t lo, int64_t hi) {
         for (int64_t g = lo; g < hi; ++g) {
@@ -405,10 +426,10 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     // The group sweep holds costs times 16 with a tag (group.hip) and runs up to half its widest band past the
     // corner: every value it makes stays under 2^30.
     // The group and block sweeps hold one gap piece: two-piece batches take the wave sweeps alone.
-    b->groups_fit = !b->parsed.two_piece && group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
+    b->groups_fit = !b->parsed.two_piece && !b->free_ends && group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
                     ((int64_t)b->group_max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
     b->block_mask = 0;
-    for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES && !b->parsed.two_piece; ++cls)
+    for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES && !b->parsed.two_piece && !b->free_ends; ++cls)
         if (block_letters(b->max_letters, cls) <= b->lds_limit &&
             ((int64_t)b->max_letters + class_width(cls) + 8) * b->dearest < REAL_LIMIT &&
             !(block_s(cls) > 8 && b->kind == DINARA_BATCH_ALIGNMENTS))  // the widest keep no decisions
@@ -418,6 +439,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     b->first_low.resize(slots);
     b->first_high.resize(slots);
     for (auto &l : b->first_lists) l.clear();
+    if (b->free_ends) b->a_free.assign((size_t)slots * 4, 0);
     b->scratch_base.resize(slots + 1);
     int64_t scratch = 0, text = 0;
     for (int64_t s = 0; s < slots; ++s) {
@@ -427,6 +449,19 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         if (b->user_band) {
             lo = std::max(lo, b->user_low);
             hi = std::min(hi, b->user_high);
+        }
+        if (b->free_ends) {
+            // The forward phase: Ties.LEFT over the pair as it is, Ties.RIGHT over both sequences reversed with each
+            // sequence's free letters swapped end for end (reference tests/test_alignment.mojo:2054-2093). Its
+            // band holds every diagonal a path may start or end on, and one either side.
+            const bool right = b->options.right_ties;
+            const int64_t e[4] = {right ? b->ends[1] : b->ends[0], right ? b->ends[0] : b->ends[1],
+                                  right ? b->ends[3] : b->ends[2], right ? b->ends[2] : b->ends[3]};
+            const int32_t f[4] = {(int32_t)std::min<int64_t>(e[0], n), (int32_t)std::min<int64_t>(e[1], n),
+                                  (int32_t)std::min<int64_t>(e[2], m), (int32_t)std::min<int64_t>(e[3], m)};
+            for (int k = 0; k < 4; ++k) b->a_free[4 * s + k] = f[k];
+            lo = std::max(std::min(-f[0], end - f[3]) - 1, -n - 1);
+            hi = std::min(std::max(f[2], end + f[1]) + 1, m + 1);
         }
         b->first_low[s] = lo;
         b->first_high[s] = hi;
@@ -452,10 +487,25 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     }
     b->first_offset[CLASSES] = (int64_t)first_all.size();
     if (first_all.empty()) first_all.push_back(0);  // every pair too wide: the device list still needs room
+    if (b->free_ends) {
+        // The forward phase's lists and bands, uploaded again at each run (the anchored phase writes its own).
+        b->a_first_low = b->first_low;
+        b->a_first_high = b->first_high;
+        b->a_first_all = first_all;
+        b->a_first_offset = b->first_offset;
+        b->a_rows = b->rows;
+        b->a_cols = b->cols;
+        b->a_at.assign(std::max<int64_t>(slots, 1), 0);
+        const size_t cap = (size_t)std::max<int64_t>(slots, 1);
+        if (!b->view_rows_d.reserve(cap * 4) || !b->view_cols_d.reserve(cap * 4) || !b->view_ref_at_d.reserve(cap * 4) ||
+            !b->view_qry_at_d.reserve(cap * 4) || !b->free_d.reserve(cap * 16) || !b->end_row_d.reserve(cap * 4) ||
+            !b->end_col_d.reserve(cap * 4))
+            return DINARA_OUT_OF_MEMORY;
+    }
 
     // Upload.
     DeviceContext *ctx = b->ctx;
-    if (!b->tape.upload(ctx, tape.get(), (size_t)std::max<int64_t>(words, 1) * 4) ||
+    if (!b->tape.upload(ctx, tape.get(), (size_t)padded * 4) ||
         !b->group_base.upload(ctx, gbase.data(), gbase.size() * 8) ||
         !b->group_ref_words.upload(ctx, gref.data(), gref.size() * 4) ||
         !b->rows_d.upload(ctx, b->rows.data(), std::max<int64_t>(slots, 1) * 4) ||
@@ -522,6 +572,22 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.cigar_offset = static_cast<int64_t *>(b->cigar_offset.ptr);
     a.cigar_length = static_cast<int32_t *>(b->cigar_length.ptr);
     a.retry_capacity = (int)std::max<size_t>(b->slot_pair.size(), 1);
+    a.free_ends = b->free_ends ? 1 : 0;
+    if (b->free_ends) {
+        // The forward phase reads the pair as Ties.RIGHT's mirror or as it is; the anchored phase the other way,
+        // as a global alignment's tape is (reference lanes.mojo:1523-1525).
+        const bool right = b->options.right_ties;
+        a.view_reversed = b->phase == 2 ? !right : right;
+        a.lowest_end = b->phase == 2;
+        a.record = b->kind == DINARA_BATCH_ALIGNMENTS && b->phase == 2;
+        a.view_rows = static_cast<const int32_t *>(b->view_rows_d.ptr);
+        a.view_cols = static_cast<const int32_t *>(b->view_cols_d.ptr);
+        a.view_ref_at = static_cast<const int32_t *>(b->view_ref_at_d.ptr);
+        a.view_qry_at = static_cast<const int32_t *>(b->view_qry_at_d.ptr);
+        a.free = static_cast<const int32_t *>(b->free_d.ptr);
+        a.end_row = static_cast<int32_t *>(b->end_row_d.ptr);
+        a.end_col = static_cast<int32_t *>(b->end_col_d.ptr);
+    }
     return a;
 }
 
@@ -671,7 +737,7 @@ EditArgs edit_args(dinara_batch *b) {
     e.walk_early = static_cast<int32_t *>(b->walk_early.ptr);
     e.deferred = static_cast<int32_t *>(b->deferred.ptr);
     e.deferred_count = static_cast<int32_t *>(b->deferred_count.ptr);
-    e.arena = static_cast<uint8_t *>(b->edit_arena.ptr);
+    e.arena = static_cast<uint8_t *>(const_cast<void *>(b->arena_ptr));
     e.arena_used = static_cast<unsigned long long *>(b->arena_used.ptr);
     e.arena_capacity = b->arena_capacity;
     e.stats = static_cast<uint32_t *>(b->edit_stats.ptr);
@@ -702,6 +768,7 @@ size_t arena_bytes(const dinara_batch *b) {
     }
     size_t free_bytes = 0, total = 0;
     (void)hipMemGetInfo(&free_bytes, &total);
+    if (b->ctx->edit_arena) free_bytes += b->ctx->edit_arena->size;  // the arena itself is there to be used
     const size_t most = free_bytes / 10 * 6;
     if (b->keep_first) bound *= 3;  // every attempt keeps its decisions
     return std::max<size_t>(std::min(bound, most), size_t(1) << 20);
@@ -809,13 +876,23 @@ int64_t run_unit(dinara_batch *b, bool trace) {
         b->ran = true;
         return 0;
     }
-    if (b->kind == DINARA_BATCH_ALIGNMENTS && !b->edit_arena.ptr) {
-        const size_t bytes = arena_bytes(b);
-        if (!b->edit_arena.reserve(bytes)) return DINARA_OUT_OF_MEMORY;
-        b->arena_capacity = bytes;
-        if (b->edit_graph) {
-            (void)hipGraphExecDestroy(b->edit_graph);
-            b->edit_graph = nullptr;
+    if (b->kind == DINARA_BATCH_ALIGNMENTS) {
+        // The context's arena, grown when this batch wants more than it holds; the graph is made again when the
+        // arena it was captured with moved or this batch's share of it changed.
+        if (!ctx->edit_arena) ctx->edit_arena = new DeviceBuffer();
+        if (!b->arena_want) b->arena_want = arena_bytes(b);
+        const size_t bytes = b->arena_want;
+        if (ctx->edit_arena->size < bytes) {
+            ctx->edit_arena->release();
+            if (!ctx->edit_arena->reserve(bytes)) return DINARA_OUT_OF_MEMORY;
+        }
+        if (b->arena_ptr != ctx->edit_arena->ptr || b->arena_capacity != bytes) {
+            b->arena_ptr = ctx->edit_arena->ptr;
+            b->arena_capacity = bytes;
+            if (b->edit_graph) {
+                (void)hipGraphExecDestroy(b->edit_graph);
+                b->edit_graph = nullptr;
+            }
         }
     }
     const EditArgs a = edit_args(b);
@@ -902,37 +979,25 @@ int64_t run_unit(dinara_batch *b, bool trace) {
     return 0;
 }
 
-int64_t run(dinara_batch *b) {
+// Sweeps every slot of the first lists until each is settled or left unsettled: want bands from first_low_d /
+// first_high_d, round 0 the first lists, each later round the lists the round before filed, kept on the device,
+// two sets taking turns. Only the counts come back to the host. Caller holds the device lock.
+int64_t sweep_rounds(dinara_batch *b, size_t &arena_index) {
     DeviceContext *ctx = b->ctx;
-    std::lock_guard<std::mutex> serial(ctx->lock);
-    DeviceGuard guard(ctx);
-    (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
-    if (b->unit) return run_unit(b, true);
     const int64_t slots = (int64_t)b->slot_pair.size();
-    b->launches = b->first_proven = 0;
-    b->run_ms = b->sweep_ms = b->walk_ms = 0;
-    for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_pairs[c] = 0;
-    if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
-    if (slots > 0) {
-        bool ok = hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice,
-                                 ctx->stream) == hipSuccess &&
-                  hipMemcpyAsync(b->want_high.ptr, b->first_high_d.ptr, slots * 4, hipMemcpyDeviceToDevice,
-                                 ctx->stream) == hipSuccess &&
-                  hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream) == hipSuccess;
-        if (b->kind == DINARA_BATCH_ALIGNMENTS)
-            ok = ok && hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream) == hipSuccess;
-        if (!ok) return DINARA_OUT_OF_MEMORY;
-    }
-    // Round 0 sweeps the first lists (uploaded once); each later round sweeps the lists the round before filed,
-    // kept on the device, two sets taking turns. Only the counts come back to the host.
+    if (slots == 0) return 0;
+    if (hipMemcpyAsync(b->want_low.ptr, b->first_low_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->want_high.ptr, b->first_high_d.ptr, slots * 4, hipMemcpyDeviceToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream) != hipSuccess)
+        return DINARA_OUT_OF_MEMORY;
     int64_t counts[CLASSES] = {};
-    if (slots > 0)
-        for (int c = 0; c < CLASSES; ++c) counts[c] = b->first_offset[c + 1] - b->first_offset[c];
+    for (int c = 0; c < CLASSES; ++c) counts[c] = b->first_offset[c + 1] - b->first_offset[c];
     int32_t *lists = static_cast<int32_t *>(b->retry_lists.ptr);
     int32_t *count_d = static_cast<int32_t *>(b->retry_counts.ptr);
     const size_t capacity = (size_t)std::max<int64_t>(slots, 1);
-    size_t arena_index = 0;
-    for (int round = 0; slots > 0 && round < 64; ++round) {
+    for (int round = 0; round < 64; ++round) {
         int64_t pending = 0;
         for (int c = 0; c < CLASSES; ++c) pending += counts[c];
         if (pending == 0) break;
@@ -969,7 +1034,133 @@ int64_t run(dinara_batch *b) {
             counts[c] = filed[c];
             retried += filed[c];
         }
-        if (round == 0) b->first_proven = (double)(slots - retried);
+        if (round == 0 && b->phase <= 1) b->first_proven = (double)(slots - retried);
+    }
+    return 0;
+}
+
+// The forward phase's views, free letters, bands and lists onto the device. Caller holds the device lock.
+int64_t free_phase_a(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    const size_t slots = b->slot_pair.size();
+    if (slots == 0) return 0;
+    b->phase = 1;
+    b->first_offset = b->a_first_offset;
+    if (!b->view_rows_d.write(ctx, b->a_rows.data(), slots * 4) || !b->view_cols_d.write(ctx, b->a_cols.data(), slots * 4) ||
+        !b->view_ref_at_d.write(ctx, b->a_at.data(), slots * 4) || !b->view_qry_at_d.write(ctx, b->a_at.data(), slots * 4) ||
+        !b->free_d.write(ctx, b->a_free.data(), slots * 16) ||
+        !b->first_low_d.write(ctx, b->a_first_low.data(), slots * 4) ||
+        !b->first_high_d.write(ctx, b->a_first_high.data(), slots * 4) ||
+        !b->first_lists_d.upload(ctx, b->a_first_all.data(), b->a_first_all.size() * 4))
+        return DINARA_OUT_OF_MEMORY;
+    return 0;
+}
+
+// The anchored phase of free-end alignments, from the forward phase's results: each settled pair (within the cap)
+// sweeps the letters before its end, back from the end (no free start), free to end where the forward phase was
+// free to start, its band the one its known cost proves; a pair it settles is walked from the end it finds.
+// Caller holds the device lock.
+int64_t free_phase_b(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    b->a_status_h.resize(slots);
+    b->a_cost_h.resize(slots);
+    b->a_end_row_h.resize(slots);
+    b->a_end_col_h.resize(slots);
+    if (!b->status.read(ctx, b->a_status_h.data(), slots * 4) || !b->cost.read(ctx, b->a_cost_h.data(), slots * 4) ||
+        !b->end_row_d.read(ctx, b->a_end_row_h.data(), slots * 4) ||
+        !b->end_col_d.read(ctx, b->a_end_col_h.data(), slots * 4))
+        return DINARA_OUT_OF_MEMORY;
+    const CostParams &c = b->costs;
+    auto gap = [&](int64_t k, int open, int ext, int open2, int ext2) -> int64_t {
+        if (k <= 0) return 0;
+        const int64_t one = (int64_t)open + k * ext;
+        return c.pieces == 2 ? std::min(one, (int64_t)open2 + k * ext2) : one;
+    };
+    const bool right = b->options.right_ties;
+    const int64_t cap = b->options.max_cost >= 0 ? b->options.max_cost : INT64_MAX / 4;
+    std::vector<int32_t> rows(slots), cols(slots), ref_at(slots), qry_at(slots), fr((size_t)slots * 4, 0);
+    b->b_rows_h.assign(slots, 0);
+    b->b_cols_h.assign(slots, 0);
+    for (auto &l : b->first_lists) l.clear();
+    for (int64_t s = 0; s < slots; ++s) {
+        const int32_t n = b->rows[s], m = b->cols[s];
+        const int32_t re = b->a_end_row_h[s], ce = b->a_end_col_h[s];
+        const bool go = b->a_status_h[s] == 0 && b->a_cost_h[s] <= cap;
+        rows[s] = go ? re : 0;
+        cols[s] = go ? ce : 0;
+        ref_at[s] = go && right ? n - re : 0;
+        qry_at[s] = go && right ? m - ce : 0;
+        b->b_rows_h[s] = rows[s];
+        b->b_cols_h[s] = cols[s];
+        if (!go) continue;
+        // Free to end where the forward phase was free to start.
+        const int32_t er = std::min<int32_t>(b->a_free[4 * s], re), ec = std::min<int32_t>(b->a_free[4 * s + 2], ce);
+        fr[4 * s + 1] = er;
+        fr[4 * s + 3] = ec;
+        const int64_t least = b->a_cost_h[s], target = least + 1, end = (int64_t)ce - re;
+        auto off = [&](int64_t k) -> int64_t {  // off_band_ends with no free start
+            const int64_t high = std::max<int64_t>(0, end + er), low = std::min<int64_t>(0, end - ec);
+            if (k > high) return gap(k, c.ins_open, c.ins_ext, c.ins_open2, c.ins_ext2) +
+                                 gap(k - (end + er), c.del_open, c.del_ext, c.del_open2, c.del_ext2);
+            if (k < low) return gap(-k, c.del_open, c.del_ext, c.del_open2, c.del_ext2) +
+                                gap(end - ec - k, c.ins_open, c.ins_ext, c.ins_open2, c.ins_ext2);
+            return 0;
+        };
+        int64_t h = std::max<int64_t>(0, end + er), l = std::min<int64_t>(0, end - ec);
+        while (h < ce && off(h + 1) < target) ++h;
+        while (l > -re && off(l - 1) < target) --l;
+        b->first_low[s] = (int32_t)l;
+        b->first_high[s] = (int32_t)h;
+        const int cls = first_class_for((int)(h - l + 1), false, 0, false);
+        if (cls != CLASS_TOO_WIDE) b->first_lists[cls].push_back((int32_t)s);
+    }
+    std::vector<int32_t> first_all;
+    b->first_offset.assign(CLASSES + 1, 0);
+    for (int k = 0; k < CLASSES; ++k) {
+        b->first_offset[k] = (int64_t)first_all.size();
+        first_all.insert(first_all.end(), b->first_lists[k].begin(), b->first_lists[k].end());
+    }
+    b->first_offset[CLASSES] = (int64_t)first_all.size();
+    if (first_all.empty()) first_all.push_back(0);
+    if (!b->view_rows_d.write(ctx, rows.data(), slots * 4) || !b->view_cols_d.write(ctx, cols.data(), slots * 4) ||
+        !b->view_ref_at_d.write(ctx, ref_at.data(), slots * 4) || !b->view_qry_at_d.write(ctx, qry_at.data(), slots * 4) ||
+        !b->free_d.write(ctx, fr.data(), slots * 16) || !b->first_low_d.write(ctx, b->first_low.data(), slots * 4) ||
+        !b->first_high_d.write(ctx, b->first_high.data(), slots * 4) ||
+        !b->first_lists_d.upload(ctx, first_all.data(), first_all.size() * 4))
+        return DINARA_OUT_OF_MEMORY;
+    b->phase = 2;
+    return 0;
+}
+
+int64_t run(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    std::lock_guard<std::mutex> serial(ctx->lock);
+    DeviceGuard guard(ctx);
+    (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
+    if (b->unit) return run_unit(b, true);
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    b->launches = b->first_proven = 0;
+    b->run_ms = b->sweep_ms = b->walk_ms = 0;
+    for (int c = 0; c < CLASSES; ++c) b->class_ms[c] = b->class_launches[c] = b->class_pairs[c] = 0;
+    if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    if (slots > 0 && b->kind == DINARA_BATCH_ALIGNMENTS &&
+        hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream) != hipSuccess)
+        return DINARA_OUT_OF_MEMORY;
+    size_t arena_index = 0;
+    if (b->free_ends) {
+        // Free ends: the forward phase finds each pair's cost and end (reference tests/test_alignment.mojo:1983-2000);
+        // alignments sweep again from that end, anchored, keeping decisions, for the start and the walk.
+        if (const int64_t code = free_phase_a(b)) return code;
+        b->phase = 1;
+        if (const int64_t code = sweep_rounds(b, arena_index)) return code;
+        if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
+            if (const int64_t code = free_phase_b(b)) return code;
+            b->phase = 2;
+            if (const int64_t code = sweep_rounds(b, arena_index)) return code;
+        }
+    } else if (const int64_t code = sweep_rounds(b, arena_index)) {
+        return code;
     }
     if (b->kind == DINARA_BATCH_ALIGNMENTS && slots > 0) {
         SweepArgs a = sweep_args(b);
@@ -1001,6 +1192,13 @@ int64_t fetch(dinara_batch *b) {
     if (slots == 0) return 0;
     if (!b->cost.read(ctx, b->cost_h.data(), slots * 4) || !b->status.read(ctx, b->status_h.data(), slots * 4))
         return DINARA_OUT_OF_MEMORY;
+    if (b->free_ends && b->kind == DINARA_BATCH_ALIGNMENTS) {
+        b->b_end_row_h.resize(slots);
+        b->b_end_col_h.resize(slots);
+        if (!b->end_row_d.read(ctx, b->b_end_row_h.data(), slots * 4) ||
+            !b->end_col_d.read(ctx, b->b_end_col_h.data(), slots * 4))
+            return DINARA_OUT_OF_MEMORY;
+    }
     if (b->kind == DINARA_BATCH_ALIGNMENTS) {
         unsigned long long used = 0;
         b->cigar_offset_h.resize(slots);
@@ -1064,7 +1262,19 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     b->user_high = (int)std::min<int64_t>(-o.band_low, clamp);
     // Unit costs times a scale, with no user band, take the bit-parallel path (DINARA_GOTOH_UNIT=1 keeps them on the
     // gap-cost sweeps, for comparison).
-    b->unit = !b->user_band && !parsed.two_piece && parsed.opening == 0 && parsed.del_opening == 0 &&
+    b->free_ends = free_mode(mode);
+    if (b->free_ends) {
+        // Free ends under a user band are not served (the band would move with every view).
+        if (b->user_band) {
+            delete b;
+            return fail(DINARA_INVALID_MODE);
+        }
+        b->ends[0] = mode->reference_start;
+        b->ends[1] = mode->reference_end;
+        b->ends[2] = mode->query_start;
+        b->ends[3] = mode->query_end;
+    }
+    b->unit = !b->user_band && !b->free_ends && !parsed.two_piece && parsed.opening == 0 && parsed.del_opening == 0 &&
               parsed.extension == parsed.mismatch && parsed.del_extension == parsed.mismatch &&
               !(std::getenv("DINARA_GOTOH_UNIT") && std::atoi(std::getenv("DINARA_GOTOH_UNIT")) != 0);
     b->unit_scale = (int)parsed.mismatch;
@@ -1169,13 +1379,19 @@ int64_t dinara_batch_alignments(dinara_batch *b, dinara_alignment *alignments, i
             dinara_alignment &out = alignments[p];
             out.cigar = nullptr;
             out.cigar_length = 0;
-            if (b->status_h[s] != 0) {
-                statuses[p] = unsettled_code(b->status_h[s]);
+            // Free ends: the cost is the forward phase's, the CIGAR the anchored phase's.
+            const int32_t status = b->free_ends ? b->a_status_h[s] : b->status_h[s];
+            if (status != 0) {
+                statuses[p] = unsettled_code(status);
                 continue;
             }
-            const int64_t cost = b->cost_h[s];
+            const int64_t cost = b->free_ends ? b->a_cost_h[s] : b->cost_h[s];
             if (capped && cost > b->options.max_cost) {
                 statuses[p] = DINARA_ABOVE_MAX;
+                continue;
+            }
+            if (b->free_ends && b->status_h[s] != 0) {
+                statuses[p] = unsettled_code(b->status_h[s]);
                 continue;
             }
             const int64_t length = b->cigar_length_h[s];
@@ -1193,6 +1409,23 @@ int64_t dinara_batch_alignments(dinara_batch *b, dinara_alignment *alignments, i
             out.reference_end = b->reference_length[p];
             out.query_start = 0;
             out.query_end = b->query_length[p];
+            if (b->free_ends) {
+                // The forward phase's end and the anchored phase's, as spans of the pair (Ties.RIGHT's mirrored).
+                const int64_t n = b->rows[s], m = b->cols[s];
+                const int64_t re = b->a_end_row_h[s], ce = b->a_end_col_h[s];
+                const int64_t rb = b->b_end_row_h[s], cb = b->b_end_col_h[s];
+                if (b->options.right_ties) {
+                    out.reference_start = n - re;
+                    out.reference_end = n - re + rb;
+                    out.query_start = m - ce;
+                    out.query_end = m - ce + cb;
+                } else {
+                    out.reference_start = re - rb;
+                    out.reference_end = re;
+                    out.query_start = ce - cb;
+                    out.query_end = ce;
+                }
+            }
             out.cigar = text;
             out.cigar_length = length;
             statuses[p] = 0;
@@ -1496,11 +1729,7 @@ int64_t dinara_batch_set_keep_first(dinara_batch *b, int64_t on) {
     if (!b) return DINARA_INVALID_MODE;
     b->keep_first = on != 0;
     drop_graph(b);
-    if (b->edit_arena.ptr) {  // its size follows the setting: made again at the next run
-        DeviceGuard guard(b->ctx);
-        b->edit_arena.release();
-        b->arena_capacity = 0;
-    }
+    b->arena_want = 0;  // the arena it asks for follows the setting: worked out again at the next run
     return 0;
 }
 
@@ -1668,7 +1897,7 @@ int64_t carved_unit_distances(Carve *carve, int64_t gpu, int64_t pairs, const ch
     ParsedCosts parsed = parse_costs(costs);
     if (parsed.code) return parsed.code;
     if (const int64_t code = check_mode(mode)) return code;
-    if (parsed.two_piece) return DINARA_INVALID_MODE;
+    if (parsed.two_piece || free_mode(mode)) return DINARA_INVALID_MODE;
     const Options o = parse_options(options);
     const bool unit = parsed.opening == 0 && parsed.del_opening == 0 && parsed.extension == parsed.mismatch &&
                       parsed.del_extension == parsed.mismatch && o.band_low <= -UNBOUNDED && o.band_high >= UNBOUNDED;

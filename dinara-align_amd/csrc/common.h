@@ -24,6 +24,10 @@ struct CostParams {
     int del_first2, ins_first2;
 };
 
+// Zero words after the tape's last group: a letter cursor may read two words past its sequence (64 words a word
+// index across the transposed groups).
+constexpr int TAPE_PADDING_WORDS = 4 * 64;
+
 // Which pairs a launch takes and where their letters lie. The tape holds every sequence 2 or 8 bits a
 // letter, 16 or 4 to a 32-bit word, transposed by groups of 64 slots: slot s (the s-th pair in length
 // order) has word w of its reference at tape[group_base[s / 64] + w * 64 + s % 64], its query words after
@@ -122,6 +126,17 @@ struct SweepArgs {
     int32_t *retry_lists;  // CLASSES lists of `retry_capacity` slots each
     int32_t *retry_counts;
     int retry_capacity;
+    // Free ends (wave sweeps only). Each slot sweeps a view of its pair: view_rows reference letters from
+    // view_ref_at and view_cols query letters from view_qry_at, read back to front when `view_reversed`; free[4 s ..
+    // 4 s + 3] the letters it may leave out for nothing at the start and end of the view's reference, then at those
+    // of its query. A settled slot's end cell (the least cost; ties to the highest diagonal, or the lowest with
+    // `lowest_end`) goes to end_row / end_col.
+    int free_ends;
+    int view_reversed;
+    int lowest_end;
+    const int32_t *view_rows, *view_cols, *view_ref_at, *view_qry_at;
+    const int32_t *free;
+    int32_t *end_row, *end_col;
 };
 
 // Host-side entry points of sweep.hip.

@@ -25,6 +25,10 @@ struct DeviceContext {
     hipStream_t side(size_t i);
     hipEvent_t mark(size_t i);
     std::mutex lock;  // one caller's work on the stream at a time (the events are shared)
+    // The unit-cost path's decision arena, lent to each alignment run under `lock` (a run walks its decisions before
+    // it returns, so no batch holds it between runs): allocated once, grown when a run wants more, kept until the
+    // process ends, so a fresh batch does not pay for mapping tens of gigabytes.
+    struct DeviceBuffer *edit_arena = nullptr;
 };
 
 // The context of GPU `gpu`, made on first use; null when there is no such GPU.

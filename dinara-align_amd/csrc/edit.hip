@@ -13,10 +13,11 @@
 namespace dinara {
 namespace {
 
-// Lane t receives lane t - 1's value (FROM_LOWER) or lane t + 1's; the wave's end lane keeps its own.
+// Lane t receives lane t - 1's value (FROM_LOWER) or lane t + 1's; the wave's end lane receives 0 (its callers
+// replace the end lane's value, so no copy of the old value is needed).
 template <bool FROM_LOWER>
 __device__ __forceinline__ uint32_t wave_neighbour(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, FROM_LOWER ? 0x138 : 0x130, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, FROM_LOWER ? 0x138 : 0x130, 0xF, 0xF, true);
 }
 
 // The even bits of a 32-bit word packed into its low 16.
@@ -63,35 +64,61 @@ __device__ __forceinline__ void load_planes(const uint32_t *text, int n, int fir
     }
 }
 
-// One sequence read a letter at a time from letter `from` >= 0 on: the code of letter `at` (0 past n - 1), then
-// the next. The word after the current one is loaded a word ahead, so a column never waits on its own load (and,
-// since stores count against the same counter, never on the stores before it either).
+// One sequence read a letter at a time from letter `from` >= 0 on: its code, then the next's. `cur` holds the current
+// word shifted so its next code is in the low bits; the word after it is loaded a word ahead, so a column never waits
+// on its own load. The refill load has no bounds check, so its result lands straight in the register it is kept in
+// (a checked load's result would be copied there at once, the copy waiting for the load): a cursor reads at most two
+// words past its sequence's last, which the tape's padding and the next group's words hold, and letters past the end
+// are never used (columns stop at the query's end; rows below the reference's feed no row above them). A lane with
+// no pair is parked: it never loads. All unsigned: no sign fix-ups in the column step.
 template <int BITS>
 struct Cursor {
-    static constexpr int PER = 32 / BITS;
+    static constexpr uint32_t PER = 32 / BITS;
+    static constexpr uint32_t MASK = BITS == 8 ? 0xFFu : 3u;
     const uint32_t *text;
-    int n, at;
-    uint32_t cur, ahead;
+    uint32_t cur, ahead, left, next_word;
 
-    __device__ uint32_t word(int w) const { return w * PER < n ? text[(size_t)w * 64] : 0u; }
-    __device__ void begin(const uint32_t *t, int length, int from) {
+    __device__ void park() {
+        text = nullptr;
+        cur = ahead = 0;
+        left = 0xFFFFFFFFu;  // never refills
+        next_word = 0;
+    }
+    __device__ void begin(const uint32_t *t, int from) {
         text = t;
-        n = length;
-        at = from;
-        cur = word(at / PER);
-        ahead = word(at / PER + 1);
+        const uint32_t w = (uint32_t)from / PER, r = (uint32_t)from % PER;
+        cur = text[(size_t)w * 64] >> (BITS * r);
+        left = PER - r;
+        ahead = text[(size_t)(w + 1) * 64];
+        next_word = w + 2;
     }
     __device__ uint32_t next() {
-        const uint32_t code = (cur >> (BITS * (at % PER))) & ((BITS == 8) ? 0xFFu : 3u);
-        ++at;
-        if (at % PER == 0) {
+        const uint32_t code = cur & MASK;
+        cur >>= BITS;
+        if (--left == 0) {
             cur = ahead;
-            ahead = word(at / PER + 1);
+            left = PER;
+            ahead = text[(size_t)next_word * 64];
+            ++next_word;
         }
         return code;
     }
 };
 
+
+// A plane of K 64-bit words shifted down one bit (bit i takes bit i + 1), each word's bit 63 taken from the next
+// word's bit 0 and the last word's from bit 0 of `incoming`: two alignbits a word.
+template <int K>
+__device__ __forceinline__ void slide_down(uint64_t (&x)[K], uint32_t incoming) {
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+        const uint32_t lo = (uint32_t)x[kk], hi = (uint32_t)(x[kk] >> 32);
+        const uint32_t in = kk + 1 < K ? (uint32_t)x[kk + 1 < K ? kk + 1 : kk] : incoming;
+        const uint32_t new_lo = __builtin_amdgcn_alignbit(hi, lo, 1);
+        const uint32_t new_hi = __builtin_amdgcn_alignbit(in, hi, 1);
+        x[kk] = ((uint64_t)new_hi << 32) | new_lo;
+    }
+}
 
 // Where the HOLD columns HOLD q .. HOLD q + HOLD - 1 of word w of a pair's `words` words lie in its kept decisions:
 // eight columns of a word take 128 contiguous bytes (two 64-bit planes a column), the words of an eight-column block
@@ -190,8 +217,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     int n = 0, m = 0, lo = 0, hi = 0, j = 0, top = 0;
     uint64_t pv[K], mv[K], pl[BITS][K];
     Cursor<BITS> bottom, query;
-    bottom.begin(nullptr, 0, 0);
-    query.begin(nullptr, 0, 0);
+    bottom.park();
+    query.park();
     uint8_t *trace = nullptr;
     bool recording = false;    // this group keeps the current pair's decisions
     long long trace_at = 0;    // and where, from the arena's start
@@ -252,8 +279,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 #pragma unroll
                         for (int b = 0; b < BITS; ++b) pl[b][kk] = planes[b];
                     }
-                    bottom.begin(ref, n, -lo);
-                    query.begin(qry, m, 0);
+                    bottom.begin(ref, -lo);
+                    query.begin(qry, 0);
                     if (TRACE) {
                         trace_at = a.trace_offset[slot];
                         recording = true;
@@ -362,7 +389,11 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     }
                 }
             }
-            if (finishing) slot = -1;
+            if (finishing) {
+                slot = -1;
+                bottom.park();
+                query.park();
+            }
             continue;
         }
 
@@ -373,27 +404,42 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             const int other = __shfl_xor(steps, d);
             steps = other < steps ? other : steps;
         }
+        // Every lane holds the same count: a scalar loop counter.
+        steps = __builtin_amdgcn_readfirstlane(steps);
         const bool active = slot >= 0;
         for (int step = 0; step < steps; ++step) {
         // One column. The window moves down a row: each word takes the bit of the word below; the band's bottom
         // takes +1 under it (Pv), and the new bottom row's letter.
         ++j;
-        {
+        if constexpr (BITS == 2) {
+            // Each plane slides down a row: its top word's bit 63 comes from bit 0 of the next lane's lowest word (one
+            // DPP move a plane), or at the band's bottom from the stand-ins (+1 vertical, no -1) and the new bottom
+            // row's letter; two alignbits shift a 64-bit word and take that bit in.
+            uint32_t ipv = L > 1 ? wave_neighbour<false>((uint32_t)pv[0]) : 0u;
+            uint32_t imv = L > 1 ? wave_neighbour<false>((uint32_t)mv[0]) : 0u;
+            uint32_t ip0 = L > 1 ? wave_neighbour<false>((uint32_t)pl[0][0]) : 0u;
+            uint32_t ip1 = L > 1 ? wave_neighbour<false>((uint32_t)pl[1][0]) : 0u;
+            if (last_lane) {
+                const uint32_t code = bottom.next();
+                ipv = 1u;
+                imv = 0u;
+                ip0 = code;
+                ip1 = code >> 1;
+            }
+            slide_down<K>(pv, ipv);
+            slide_down<K>(mv, imv);
+            slide_down<K>(pl[0], ip0);
+            slide_down<K>(pl[1], ip1);
+        } else {
             uint32_t low = (uint32_t)(pv[0] & 1) | ((uint32_t)(mv[0] & 1) << 1);
 #pragma unroll
             for (int b = 0; b < BITS; ++b) low |= (uint32_t)(pl[b][0] & 1) << (2 + b);
             uint32_t in = L > 1 ? wave_neighbour<false>(low) : 0u;
             if (last_lane) in = 1u | (bottom.next() << 2);
+            slide_down<K>(pv, in);
+            slide_down<K>(mv, in >> 1);
 #pragma unroll
-            for (int kk = 0; kk < K; ++kk) {
-                const bool top_word = kk == K - 1;
-                pv[kk] = (pv[kk] >> 1) | ((top_word ? (uint64_t)(in & 1) : pv[kk + (top_word ? 0 : 1)] & 1) << 63);
-                mv[kk] = (mv[kk] >> 1) | ((top_word ? (uint64_t)((in >> 1) & 1) : mv[kk + (top_word ? 0 : 1)] & 1) << 63);
-#pragma unroll
-                for (int b = 0; b < BITS; ++b)
-                    pl[b][kk] = (pl[b][kk] >> 1) |
-                                ((top_word ? (uint64_t)((in >> (2 + b)) & 1) : pl[b][kk + (top_word ? 0 : 1)] & 1) << 63);
-            }
+            for (int b = 0; b < BITS; ++b) slide_down<K>(pl[b], in >> (2 + b));
         }
         // The rows whose letter equals the query's letter j - 1.
         const uint32_t c = query.next();
@@ -419,8 +465,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             all_ones = all_ones && sum[kk] == ~0ull;
         }
         if constexpr (L > 1) {
-            const unsigned long long gen = __ballot(carry && !last_lane);
-            const unsigned long long prop = __ballot(all_ones && !last_lane);
+            const unsigned long long gen = __builtin_amdgcn_ballot_w64(carry != 0 && !last_lane);
+            const unsigned long long prop = __builtin_amdgcn_ballot_w64(all_ones && !last_lane);
             const unsigned long long into = (gen + (gen | prop)) ^ prop;
             if ((into >> lane) & 1ull) {
                 uint32_t up = 1;

@@ -29,18 +29,28 @@ namespace dinara {
 // piece's, its letters cost no less), so the cells' values, and the decisions on every path the walk can take,
 // are the recurrence's. Decisions are then a byte a cell: bits 0-2 the source (0 two equal letters, 5 two unequal,
 // 1 and 2 the first piece's deletion and insertion layers, 3 and 4 the second's), bit 2 + L layer L extends.
-template <int Q, bool RECORD, int P>
+// FREE: free ends. The slot sweeps a view of its pair (SweepArgs: view_*), cells (r, 0) with r up to its free
+// leading reference letters and (0, c) with c up to its free leading query letters start an alignment for nothing,
+// and the least cost is taken over the cells it may end on: column m in the last free reference letters' rows,
+// row n in the last free query letters' columns (ties to the highest diagonal, reference letters less query
+// letters, or with `lowest_end` the lowest); reference tests/test_alignment.mojo:1944-2020.
+template <int Q, bool RECORD, int P, bool FREE>
 __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
     extern __shared__ uint8_t letters[];
     const int t = blockIdx.x;
     const int slot = a.list[t];
     const int lane = threadIdx.x;
-    const int n = a.tape.rows[slot], m = a.tape.cols[slot];
+    const int n = FREE ? a.view_rows[slot] : a.tape.rows[slot], m = FREE ? a.view_cols[slot] : a.tape.cols[slot];
     constexpr int W = 64 * Q;
     const int need_low = a.want_low[slot], need_high = a.want_high[slot];
     const int lo = need_low - (W - (need_high - need_low + 1)) / 2;
     const int end = m - n;
     const CostParams k = a.costs;
+    int fr[4] = {0, 0, 0, 0};  // free letters: reference start, end, query start, end
+    if (FREE) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) fr[f] = a.free[4 * slot + f];
+    }
 
     // Both sequences' letters into LDS, a byte each: the reference's first, then the query's.
     {
@@ -49,23 +59,42 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
         const uint32_t *qry_words = ref_words + (size_t)a.tape.group_ref_words[group] * 64;
         const int bits = a.tape.bits, per = 32 / bits;
         const uint32_t mask = (1u << bits) - 1;
-        for (int i = lane; i < n; i += 64)
-            letters[i] = (uint8_t)((ref_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
-        for (int i = lane; i < m; i += 64)
-            letters[n + i] = (uint8_t)((qry_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+        const int ref_at = FREE ? a.view_ref_at[slot] : 0, qry_at = FREE ? a.view_qry_at[slot] : 0;
+        const bool back = FREE && a.view_reversed;
+        for (int i = lane; i < n; i += 64) {
+            const int x = back ? ref_at + n - 1 - i : ref_at + i;
+            letters[i] = (uint8_t)((ref_words[(size_t)(x / per) * 64] >> (bits * (x % per))) & mask);
+        }
+        for (int i = lane; i < m; i += 64) {
+            const int x = back ? qry_at + m - 1 - i : qry_at + i;
+            letters[n + i] = (uint8_t)((qry_words[(size_t)(x / per) * 64] >> (bits * (x % per))) & mask);
+        }
     }
     __syncthreads();
 
     const int first = lo + lane * Q;  // this lane's lowest diagonal
     int H[Q], D[Q], D2[Q];
+    // Free ends: this lane's best end so far, and its lane diagonal (query letters less reference letters).
+    int best_end = INF, best_diagonal = 0;
+    auto consider = [&](int value, int kk) {
+        // A higher library diagonal (reference less query) is a lower lane diagonal.
+        const bool nearer = a.lowest_end ? kk > best_diagonal : kk < best_diagonal;
+        if (value < best_end || (value == best_end && value < INF && nearer)) {
+            best_end = value;
+            best_diagonal = kk;
+        }
+    };
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int kk = first + i;
-        int top = kk <= m ? k.ins_open + kk * k.ins_ext : INF;
-        if (P == 2 && kk <= m) top = imin(top, k.ins_open2 + kk * k.ins_ext2);
-        H[i] = kk < 0 ? INF : (kk == 0 ? 0 : top);
+        const int from = FREE ? fr[2] : 0;  // row 0: the free leading query letters cost nothing
+        int top = kk <= m ? k.ins_open + (kk - from) * k.ins_ext : INF;
+        if (P == 2 && kk <= m) top = imin(top, k.ins_open2 + (kk - from) * k.ins_ext2);
+        H[i] = kk < 0 ? INF : (kk <= from ? 0 : top);
         D[i] = INF;
         D2[i] = INF;
+        if (FREE && kk >= 0 && kk <= m && ((n == 0 && kk >= m - fr[3]) || (kk == m && n <= fr[1])))
+            consider(H[i], kk);  // row 0's cells an alignment may end on
     }
     const bool barred_band = a.user_band != 0;
     const int user_low = a.user_low, user_high = a.user_high;
@@ -82,8 +111,10 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
         int T[Q], SUB[Q], DN[Q], DN2[Q], X[Q], X2[Q];
         bool EQ[Q];
         bool ED_LE[Q], ED2_LE[Q];
-        int border = k.del_open + r * k.del_ext;
-        if (P == 2) border = imin(border, k.del_open2 + r * k.del_ext2);
+        const int skipped = FREE ? r - fr[0] : r;  // column 0: the free leading reference letters cost nothing
+        int border = k.del_open + skipped * k.del_ext;
+        if (P == 2) border = imin(border, k.del_open2 + skipped * k.del_ext2);
+        if (FREE && skipped <= 0) border = 0;
         int aggregate = INF, aggregate2 = INF;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -161,6 +192,8 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
             }
             if (barred_band && (kk < user_low || kk > user_high)) h = INF;
             HN[i] = h;
+            if (FREE && c >= 0 && c <= m && ((c == m && r >= n - fr[1]) || (r == n && c >= m - fr[3])))
+                consider(c == 0 ? T[i] : h, kk);
         }
         if (RECORD) {
             int hl_prev = __shfl_up(HN[Q - 1], 1);
@@ -213,18 +246,44 @@ __global__ __launch_bounds__(64) void sweep_wave(SweepArgs a) {
         }
     }
 
-    // The corner's cost, from the lane holding diagonal `end`.
-    int mine = INF;
+    int found;
+    if (FREE) {
+        // The wave's best end: the least cost, then the preferred diagonal.
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-        if (first + i == end) mine = H[i];
-    const int holder = (end - lo) / Q;
-    const int found = __shfl(mine, holder);
+        for (int d = 1; d < 64; d <<= 1) {
+            const int other = __shfl_xor(best_end, d), other_diagonal = __shfl_xor(best_diagonal, d);
+            const bool nearer = a.lowest_end ? other_diagonal > best_diagonal : other_diagonal < best_diagonal;
+            if (other < best_end || (other == best_end && other < INF && nearer)) {
+                best_end = other;
+                best_diagonal = other_diagonal;
+            }
+        }
+        found = best_end;
+    } else {
+        // The corner's cost, from the lane holding diagonal `end`.
+        int mine = INF;
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            if (first + i == end) mine = H[i];
+        const int holder = (end - lo) / Q;
+        found = __shfl(mine, holder);
+    }
     if (lane != 0) return;
     int want_low, want_high;
-    if (!certify(a, n, m, found, lo, lo + W - 1, &want_low, &want_high)) {
+    const bool proven = FREE ? found < INF && certify_ends(a, n, m, found, lo, lo + W - 1, fr, &want_low, &want_high)
+                             : certify(a, n, m, found, lo, lo + W - 1, &want_low, &want_high);
+    if (!proven) {
+        if (FREE && found >= INF) {  // no end in the band: the whole matrix's diagonals next
+            want_low = -n;
+            want_high = m;
+        }
         file_retry(a, slot, n + m, want_low, want_high);
         return;
+    }
+    if (FREE) {
+        // The end cell on that diagonal: column m (at or above the corner's diagonal) or row n (below it).
+        a.end_row[slot] = best_diagonal >= end ? m - best_diagonal : n;
+        a.end_col[slot] = best_diagonal >= end ? m : n + best_diagonal;
     }
     a.cost[slot] = found;
     a.status[slot] = 0;
@@ -244,7 +303,9 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
     int64_t moves = -1;
     if (settled) {
         const int cls = a.walk_class[slot];
-        const int n = a.tape.rows[slot], m = a.tape.cols[slot];
+        // Free ends: from the end the anchored phase found, back to the view's origin.
+        const int n = a.free_ends ? a.end_row[slot] : a.tape.rows[slot];
+        const int m = a.free_ends ? a.end_col[slot] : a.tape.cols[slot];
         const int lo = a.walk_low[slot];
         const uint8_t *base = reinterpret_cast<const uint8_t *>(a.walk_flags[slot]);
         if (!band_is_wave(cls)) {
@@ -284,16 +345,22 @@ __global__ __launch_bounds__(256) void walk_bands(SweepArgs a) {
 
 // ------------------------------------------------------------------ launches
 
-template <int Q>
-static void launch_wave(const SweepArgs &a, size_t lds, hipStream_t stream) {
+template <int Q, bool FREE>
+static void launch_wave_as(const SweepArgs &a, size_t lds, hipStream_t stream) {
     const dim3 grid(a.count), block(64);
     if (a.costs.pieces == 2) {
-        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 2>), grid, block, lds, stream, a);
-        else hipLaunchKernelGGL((sweep_wave<Q, false, 2>), grid, block, lds, stream, a);
+        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 2, FREE>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_wave<Q, false, 2, FREE>), grid, block, lds, stream, a);
     } else {
-        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 1>), grid, block, lds, stream, a);
-        else hipLaunchKernelGGL((sweep_wave<Q, false, 1>), grid, block, lds, stream, a);
+        if (a.record) hipLaunchKernelGGL((sweep_wave<Q, true, 1, FREE>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_wave<Q, false, 1, FREE>), grid, block, lds, stream, a);
     }
+}
+
+template <int Q>
+static void launch_wave(const SweepArgs &a, size_t lds, hipStream_t stream) {
+    if (a.free_ends) launch_wave_as<Q, true>(a, lds, stream);
+    else launch_wave_as<Q, false>(a, lds, stream);
 }
 
 int class_width(int cls) { return band_width(cls); }

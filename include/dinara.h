@@ -10,10 +10,11 @@
  * sequence of a pair is the reference, the second the query: a CIGAR's `=` is a match, `X` a
  * substitution (`M` for either without eqx), `D` a reference letter alone, `I` a query letter alone.
  *
- * What the device serves: global alignment under unit, linear or one-piece gap-affine costs, deletions
- * optionally priced apart from insertions, any band and any cost cap; and, under a substitution table,
- * global and local scores and alignments. Other modes (free ends, extension, two gap pieces) return
- * DINARA_INVALID_MODE for the whole batch: this library holds no CPU fallback.
+ * What the device serves: global alignment under unit, linear, gap-affine or two-piece costs, deletions
+ * optionally priced apart from insertions, any band and any cost cap; free ends (ends_free and its infix,
+ * prefix and suffix presets, costs minimized, no user band); and, under a substitution table, global and local
+ * scores and alignments. Other modes (extension, local or free ends under Costs with a match score, free ends
+ * under a user band) return DINARA_INVALID_MODE for the whole batch: this library holds no CPU fallback.
  */
 #ifndef DINARA_H
 #define DINARA_H
