@@ -22,12 +22,17 @@ enum ScoreMode : int {
 constexpr int SCORE_MAX_C = 8;
 constexpr int SCORE_STRIP_MAX = 64 * SCORE_MAX_C;
 
+// A reference too long to stage whole in LDS is read through a ring of this many chunks of this many letters.
+constexpr int SCORE_RING_CHUNK = 256;
+constexpr int SCORE_RING_CHUNKS = 4;
+
 struct ScoreArgs {
     const uint8_t *codes;    // every sequence's letter codes (the alphabet's places)
     const int64_t *ref_at;   // per slot: where its reference's codes start
     const int64_t *qry_at;   // per slot: where its query's codes start
     const int32_t *view;     // per slot: reference begin, reference letters, query begin, query letters
     int reversed;            // the views read both sequences back to front from their ends
+    int ring;                // the references go through an LDS ring rather than being staged whole
     const int32_t *list;     // the slots of this launch, `count` of them
     int count;
     const int8_t *table;     // table[a * 32 + b]: reference letter a against query letter b
@@ -55,8 +60,9 @@ struct ScoreArgs {
     int slots;
 };
 
-// Host-side entry points of score.hip. `lds` is the bytes a block stages: the profile, then the reference's letters.
-size_t score_lds(int letters, int rows);
+// Host-side entry points of score.hip. `lds` is the bytes a block stages: the profile, then the reference's letters
+// (or, with `ring`, the ring they pass through).
+size_t score_lds(int letters, int rows, bool ring);
 void launch_score(const ScoreArgs &a, size_t lds, hipStream_t stream);
 void launch_score_walk(const ScoreArgs &a, hipStream_t stream);
 

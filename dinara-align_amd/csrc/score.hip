@@ -63,7 +63,19 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
     const uint8_t *qry_codes = a.codes + a.qry_at[slot] + view[2];
     uint2 *profile = reinterpret_cast<uint2 *>(lds);
     uint8_t *ref = lds + (size_t)a.letters * 64 * 8;
-    for (int i = lane; i < rows; i += 64) ref[i] = a.reversed ? ref_codes[rows - 1 - i] : ref_codes[i];
+    // Four bytes of the reference's letters from `at` on, as the views read them (zero past the end).
+    auto letters4 = [&](int at) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (at + t < rows) w |= (uint32_t)(a.reversed ? ref_codes[rows - 1 - at - t] : ref_codes[at + t]) << (8 * t);
+        return w;
+    };
+    // A reference too long for LDS goes through a ring of SCORE_RING_CHUNKS chunks instead of being staged whole.
+    const bool ring = a.ring != 0;
+    const int ref_mask = ring ? SCORE_RING_CHUNKS * SCORE_RING_CHUNK - 1 : 0x7fffffff;
+    if (!ring)
+        for (int i = lane; i < rows; i += 64) ref[i] = a.reversed ? ref_codes[rows - 1 - i] : ref_codes[i];
 
     const int target = REACH ? a.target[slot] : 0;
     const int strip_cols = 64 * C;
@@ -93,6 +105,17 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
             }
             profile[letter * 64 + lane] = make_uint2(lo, hi);
         }
+        // Ring: chunk k of the letters (SCORE_RING_CHUNK of them from k * SCORE_RING_CHUNK) sits in ring place
+        // k mod SCORE_RING_CHUNKS. Step s reads letters s - 63 .. s, so while steps 256k + 1 .. 256k + 256 run
+        // chunks k - 1, k and k + 1 are held; chunk k + 1 is written at step 256k + 1, from a register loaded a
+        // range of steps before, and chunk k + 2 loaded into that register.
+        uint32_t pending = 0;
+        if (ring) {
+            static_assert(SCORE_RING_CHUNK == 4 * 64, "a chunk is four letters a lane");
+            reinterpret_cast<uint32_t *>(ref)[lane] = letters4(4 * lane);
+            reinterpret_cast<uint32_t *>(ref)[64 + lane] = letters4(SCORE_RING_CHUNK + 4 * lane);
+            pending = letters4(2 * SCORE_RING_CHUNK + 4 * lane);
+        }
         __syncthreads();
 
         int Hp[C], Dp[C];  // the row above, the lane's columns
@@ -111,10 +134,16 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
         uint2 prof = make_uint2(0, 0);
         {
             const int r = 1 - lane;
-            if (r >= 1 && r <= rows) prof = profile[ref[r - 1] * 64 + lane];
+            if (r >= 1 && r <= rows) prof = profile[ref[(r - 1) & ref_mask] * 64 + lane];
         }
         for (int s = 1; s <= steps; ++s) {
             const int r = s - lane;
+            if (ring && (s & (SCORE_RING_CHUNK - 1)) == 1 && s > 1) {
+                const int k = s / SCORE_RING_CHUNK;
+                reinterpret_cast<uint32_t *>(ref)[((k + 1) & (SCORE_RING_CHUNKS - 1)) * 64 + lane] = pending;
+                pending = letters4((k + 2) * SCORE_RING_CHUNK + 4 * lane);
+                __syncthreads();
+            }
             int lh = from_left_lane(SCORE_NEG, out_h), li = from_left_lane(SCORE_NEG, out_i);
             if (lane == 0) {
                 if (strip == 0 || r < 1 || r > rows) {
@@ -127,7 +156,7 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
             }
             // The next step's row, its profile read ahead.
             uint2 next = make_uint2(0, 0);
-            if (r + 1 >= 1 && r + 1 <= rows) next = profile[ref[r] * 64 + lane];
+            if (r + 1 >= 1 && r + 1 <= rows) next = profile[ref[r & ref_mask] * 64 + lane];
             if (r >= 1 && r <= rows) {
                 uint32_t word = 0;
                 int hl = lh, il = li, hd = left_prev;
@@ -291,7 +320,9 @@ __global__ __launch_bounds__(256) void score_walk(ScoreArgs a) {
 
 // ------------------------------------------------------------------ launches
 
-size_t score_lds(int letters, int rows) { return (size_t)letters * 64 * 8 + (size_t)rows; }
+size_t score_lds(int letters, int rows, bool ring) {
+    return (size_t)letters * 64 * 8 + (ring ? (size_t)SCORE_RING_CHUNKS * SCORE_RING_CHUNK : (size_t)rows);
+}
 
 template <int MODE>
 static void launch_mode(const ScoreArgs &a, size_t lds, hipStream_t stream) {

@@ -193,46 +193,57 @@ ScoreArgs score_args(Batch *b, int mode, int reversed) {
 
 // One pass over the slots whose views are not empty: grouped by the columns a lane takes, each group one launch (or
 // more, when a trace's decisions pass the budget), each trace launch followed by its walk.
+// The reference letters a block can stage whole in LDS beside the profile; a longer reference goes through the
+// ring.
+int64_t lds_room(const Batch *b) {
+    const size_t profile = (size_t)b->scoring.letters * 64 * 8;
+    return b->ctx->lds_limit > profile ? (int64_t)(b->ctx->lds_limit - profile) : 0;
+}
+
 int64_t run_pass(Batch *b, int mode, int reversed, const std::vector<int32_t> &slots, hipEvent_t start,
                  hipEvent_t stop, double *pass_ms, double *walk_ms) {
     DeviceContext *ctx = b->ctx;
-    std::vector<int32_t> by_c[SCORE_MAX_C + 1];
-    int64_t rows_max[SCORE_MAX_C + 1] = {0}, cols_max[SCORE_MAX_C + 1] = {0};
+    // Groups by the columns a lane takes and by whether the reference fits LDS whole (group g = c + ring * G).
+    constexpr int G = SCORE_MAX_C + 1;
+    const int64_t room = lds_room(b);
+    std::vector<int32_t> by_c[2 * G];
+    int64_t rows_max[2 * G] = {0}, cols_max[2 * G] = {0};
     for (int32_t s : slots) {
         const int32_t rows = b->view_h[4 * s + 1], cols = b->view_h[4 * s + 3];
         if (rows <= 0 || cols <= 0) continue;
-        const int c = lane_columns(cols);
-        by_c[c].push_back(s);
-        rows_max[c] = std::max<int64_t>(rows_max[c], rows);
-        cols_max[c] = std::max<int64_t>(cols_max[c], cols);
+        const int g = lane_columns(cols) + (rows > room ? G : 0);
+        by_c[g].push_back(s);
+        rows_max[g] = std::max<int64_t>(rows_max[g], rows);
+        cols_max[g] = std::max<int64_t>(cols_max[g], cols);
     }
     std::vector<int32_t> all;
-    std::vector<size_t> at(SCORE_MAX_C + 2, 0);
-    for (int c = 1; c <= SCORE_MAX_C; ++c) {
-        at[c] = all.size();
-        all.insert(all.end(), by_c[c].begin(), by_c[c].end());
+    std::vector<size_t> at(2 * G, 0);
+    for (int g = 0; g < 2 * G; ++g) {
+        at[g] = all.size();
+        all.insert(all.end(), by_c[g].begin(), by_c[g].end());
     }
-    at[SCORE_MAX_C + 1] = all.size();
     if (all.empty()) return 0;
     if (!b->list.upload(ctx, all.data(), all.size() * 4)) return failed("list upload");
     const int32_t *list = static_cast<const int32_t *>(b->list.ptr);
     ScoreArgs a = score_args(b, mode, reversed);
     hipEventRecord(start, ctx->stream);
-    for (int c = 1; c <= SCORE_MAX_C; ++c) {
-        const int64_t count = (int64_t)by_c[c].size();
+    for (int g = 0; g < 2 * G; ++g) {
+        const int64_t count = (int64_t)by_c[g].size();
         if (count == 0) continue;
+        const int c = g % G;
         a.c = c;
+        a.ring = g >= G;
         const int strip_cols = 64 * c;
-        const int64_t strips = (cols_max[c] + strip_cols - 1) / strip_cols;
-        const size_t lds = score_lds(b->scoring.letters, (int)rows_max[c]);
+        const int64_t strips = (cols_max[g] + strip_cols - 1) / strip_cols;
+        const size_t lds = score_lds(b->scoring.letters, (int)rows_max[g], a.ring != 0);
         int64_t chunk = count;
         if (mode == SCORE_TRACE) {
-            a.arena_stride = strips * (rows_max[c] + 1) * 64;
+            a.arena_stride = strips * (rows_max[g] + 1) * 64;
             chunk = std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)(ARENA_BUDGET / 4 / a.arena_stride)));
         }
-        a.carry_stride = strips > 1 ? 2 * (rows_max[c] + 1) : 0;
+        a.carry_stride = strips > 1 ? 2 * (rows_max[g] + 1) : 0;
         for (int64_t first = 0; first < count; first += chunk) {
-            a.list = list + at[c] + first;
+            a.list = list + at[g] + first;
             a.count = (int)std::min(chunk, count - first);
             a.carry = nullptr;
             if (strips > 1) {
@@ -269,9 +280,6 @@ int64_t prepare(Batch *b, const char *const *refs, const int64_t *ref_lens, cons
     b->rows.assign(pairs, 0);
     b->cols.assign(pairs, 0);
     b->host_score.assign(pairs, 0);
-    const size_t lds_room = ctx->lds_limit > (size_t)b->scoring.letters * 512
-                                ? ctx->lds_limit - (size_t)b->scoring.letters * 512
-                                : 0;
     parallel_for(pairs, b->threads, [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; ++i) {
             const int64_t n = ref_lens[i], m = qry_lens[i];
@@ -290,10 +298,6 @@ int64_t prepare(Batch *b, const char *const *refs, const int64_t *ref_lens, cons
             }
             if (!within_reach(b->scoring, n, m, b->local) || n >= (int64_t(1) << 20) || m >= (int64_t(1) << 20)) {
                 b->code[i] = DINARA_INVALID_COSTS;
-                continue;
-            }
-            if ((size_t)n > lds_room) {
-                b->code[i] = DINARA_OUT_OF_MEMORY;
                 continue;
             }
             // Against an empty side: one gap of every letter, or no local alignment (reference

@@ -189,3 +189,27 @@ def test_cigar_runs_across_chunk_edges():
     qrys = [base[:4300], base[:2047] + other[base[2047]] + base[2048:4100]]
     check_global(refs, qrys, DNA)
     check_global(refs, qrys, DNA, eqx=False)
+
+
+def test_tall_references_go_through_the_ring():
+    # A reference longer than LDS stages whole (64 KiB less the profile) is read through a ring of letters
+    # (reference tests/test_alignment.mojo:3801-3810: 70 000 letters against a 16-letter query, global and local).
+    scoring = da.Scoring.uniform(5, -4, -19, -1)
+    tall, short = "A" * 70_000, "ACGT" * 4
+    # Global: the query's four A's match, its twelve other letters mismatch, the other 69 984 letters one deletion
+    # (a gap of k letters scores opening + k extension, reference scoring.mojo:202-214).
+    assert da.score(tall, short, scoring) == 4 * 5 - 12 * 4 + (-19 - 69_984)
+    assert da.score(tall, short, scoring, da.Mode.local()) == 5
+    assert da.score(tall, "", scoring) == -19 - 70_000
+    assert da.score(tall, "", scoring, da.Mode.local()) == 0
+    rng = random.Random(16)
+    refs, qrys = [tall], [short]
+    for n, m in [(66_000, 40), (70_001, 3), (80_000, 600), (65_600, 200)]:
+        q = "".join(rng.choice("ACGT") for _ in range(m))
+        at = rng.randint(0, n - m)
+        r = "".join(rng.choice("ACGT") for _ in range(at)) + mutated(q, 0.1, rng, "ACGT")
+        refs.append(r + "".join(rng.choice("ACGT") for _ in range(n - len(r))))
+        qrys.append(q)
+    check_global(refs, qrys, scoring, eqx=False)
+    check_local(refs, qrys, scoring)
+    check_local(refs, qrys, DNA)
