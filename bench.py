@@ -60,10 +60,13 @@ CLASS_NAMES = ["sweep_group<1>", "sweep_group<2>", "sweep_group<4>", "sweep_grou
 CLASS_WIDTHS = [32, 64, 128, 256, 64, 128, 256, 512, 1024, 2048, 512, 1024, 1536, 2048, 3072, 4096, 5120, 6144, 7168,
                 8192, 12288, 16384]
 CLASSES = len(CLASS_NAMES)
-# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this bench's
-# default affine step (100,000 pairs of 250 bp), each scaled by what the same counter reads for a plain 1 GiB
-# device copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 (profiles/sweep_group_affine_100k_x_250_hbm_counters.md).
-MEASURED_TRAFFIC = {("affine", 100_000, "sweep_group<2>"): {"read": 119.1e6, "written": 823.3e6}}
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by what
+# the same counter reads for a plain 1 GiB device copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0. The affine step (100,000
+# pairs of 250 bp): profiles/sweep_group_affine_100k_x_250_hbm_counters.md. The learned run of the mixed-length
+# workload (200,000 pairs), every edit_sweep dispatch of one run summed (the bins run side by side):
+# profiles/edit_sweep_learned_uneven_200k_hbm_counters.md.
+MEASURED_TRAFFIC = {("affine", 100_000, "sweep_group<2>"): {"read": 119.1e6, "written": 823.3e6},
+                    ("uneven", 200_000, "edit_sweep"): {"read": 3.082e9, "written": 98.24e9}}
 
 
 def host_cores() -> int:
@@ -1163,10 +1166,15 @@ def run_learned_profile(args) -> int:
     prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
     prof["run_ms"] = handle.dinara_batch_stat(b, 0)
     prof["first_band_proved"] = handle.dinara_batch_stat(b, 4) / pairs
+    costs, _ = edit_results(handle, b, _lib.BATCH_ALIGNMENTS, pairs)
     handle.dinara_batch_free(b)
     handle.dinara_mlp_free(net)
+    n = np.fromiter((len(r) for r in refs), dtype=np.int64, count=pairs)
+    m = np.fromiter((len(q) for q in qrys), dtype=np.int64, count=pairs)
+    proven = float(proven_band_cells(n, m, np.maximum(costs, 0), True, opening=0, extension=1).sum())
     print(json.dumps({"profile": "learned run, mixed-length workload", "margin": args.margin, "slope": args.slope,
-                      "grid_per_cu": args.grid_per_cu, "bins": prof}))
+                      "grid_per_cu": args.grid_per_cu, "proven_band_cells": proven,
+                      "full_matrix_cells": float((n * m).sum()), "bins": prof}))
     return 0
 
 

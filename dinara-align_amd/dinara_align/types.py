@@ -120,8 +120,9 @@ class Anchor:
 @dataclass(frozen=True)
 class Mode:
     """Which alignments are allowed (reference python/dinara_align/__init__.py:166-255). This library's kernels
-    serve `GLOBAL` under Costs and `GLOBAL` or `local()` under a Scoring; every other mode is refused by the
-    library with DINARA_INVALID_MODE."""
+    serve, under Costs, `GLOBAL` and free ends (`ends_free`, `INFIX`, `PREFIX`, `SUFFIX`, no match score, no user
+    band) and, under a Scoring, `GLOBAL` or `local()`; every other mode is refused by the library with
+    DINARA_INVALID_MODE."""
 
     kind: int = KIND_ENDS_FREE
     reference_start: int = 0
