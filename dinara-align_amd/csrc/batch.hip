@@ -4,6 +4,7 @@
 // refusals), :563-819 (batch results), lanes.mojo:1163-1252 (first band), api.mojo:1188-1213 (alignments).
 
 #include <algorithm>
+#include <chrono>
 #include <functional>
 #include <atomic>
 #include <memory>
@@ -21,6 +22,7 @@
 #include "common.h"
 #include "device.h"
 #include "edit.h"
+#include "full.h"
 #include "host.h"
 #include "learned.h"
 
@@ -257,6 +259,12 @@ struct dinara_batch {
     double sweep_work[EDIT_BINS] = {}, trace_work[EDIT_BINS] = {};  // word-columns swept by each bin
     bool stats_pending = false;  // the unit path's device counters of the last run, read when first asked for
     int64_t stats_slots = 0;
+    // The whole-matrix path (full.hip) for pairs the band sweeps leave unsettled: whether any pair is long enough
+    // to need it (-1 not worked out yet), and the last run's count and time.
+    int may_need_full = -1;
+    bool whole = false;  // every pair takes the whole-matrix path, no band sweep (for checking and comparison)
+    double full_pairs = 0, full_ms = 0;
+    DeviceBuffer full_list, full_count, full_decisions, full_carry, full_claim;
 };
 
 namespace {
@@ -1133,12 +1141,97 @@ int64_t free_phase_b(dinara_batch *b) {
     return 0;
 }
 
+// The decision bytes one pair may take on the whole-matrix path: past it the pair stays unsettled.
+constexpr size_t FULL_BUDGET = size_t(16) << 30;
+
+// After a run, every pair the band sweeps left unsettled (wider than the widest band, or too many letters for the
+// kernels that stage them in LDS) is swept whole, one pair after another, and walked (full.hip); a pair under free
+// ends or a user band, or whose decisions pass FULL_BUDGET, stays unsettled. A batch none of
+// whose pairs could need it skips the pass; otherwise the unsettled slots are listed on the device and only their
+// count (and list) come back. Caller holds the device lock.
+int64_t full_pass(dinara_batch *b) {
+    DeviceContext *ctx = b->ctx;
+    const int64_t slots = (int64_t)b->slot_pair.size();
+    b->full_pairs = b->full_ms = 0;
+    if (b->free_ends || b->user_band || slots == 0) return 0;
+    if (b->may_need_full < 0) {
+        // Under both the unit bins and the band classes a band of 8192 diagonals proves any pair of fewer letters,
+        // and such a pair fits every kernel's LDS.
+        b->may_need_full = 0;
+        for (int64_t s = 0; s < slots; ++s)
+            if ((int64_t)b->rows[s] + b->cols[s] + 3 > 8192) b->may_need_full = 1;
+    }
+    if (!b->may_need_full && !b->whole) return 0;
+    const auto began = std::chrono::steady_clock::now();
+    if (!b->full_list.reserve((size_t)slots * 4) || !b->full_count.reserve(4) || !b->full_claim.reserve(8) ||
+        hipMemsetAsync(b->full_count.ptr, 0, 4, ctx->stream) != hipSuccess)
+        return DINARA_OUT_OF_MEMORY;
+    launch_full_collect(static_cast<const int32_t *>(b->status.ptr), (int)slots,
+                        static_cast<int32_t *>(b->full_list.ptr), static_cast<int32_t *>(b->full_count.ptr),
+                        ctx->stream);
+    int32_t count = 0;
+    if (hipPeekAtLastError() != hipSuccess || !b->full_count.read(ctx, &count, 4)) return DINARA_OUT_OF_MEMORY;
+    if (count == 0) return 0;
+    std::vector<int32_t> list(count);
+    if (!b->full_list.read(ctx, list.data(), (size_t)count * 4)) return DINARA_OUT_OF_MEMORY;
+    std::sort(list.begin(), list.end());
+    const bool record = b->kind == DINARA_BATCH_ALIGNMENTS;
+    const int pieces = b->costs.pieces == 2 ? 2 : 1;
+    FullArgs fa{};
+    fa.s = sweep_args(b);
+    fa.record = record ? 1 : 0;
+    for (const int32_t slot : list) {
+        const int64_t n = b->rows[slot], m = b->cols[slot];
+        if (slot < 0 || slot >= slots) continue;
+        const int64_t strips = std::max<int64_t>(1, (m + FULL_STRIP - 1) / FULL_STRIP);
+        const size_t decisions = record ? (size_t)strips * n * 64 * (pieces == 2 ? 8 : 4) : 0;
+        if (decisions > FULL_BUDGET) continue;
+        const size_t carry = (size_t)strips * (n + 1) * 8 * pieces;
+        if ((record && !b->full_decisions.reserve(decisions)) || !b->full_carry.reserve(carry) ||
+            hipMemsetAsync(b->full_carry.ptr, 0, carry, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(b->full_claim.ptr, 0, 8, ctx->stream) != hipSuccess)
+            return DINARA_OUT_OF_MEMORY;
+        fa.slot = slot;
+        fa.rows = (int)n;
+        fa.cols = (int)m;
+        fa.strips = (int)strips;
+        fa.decisions = record ? b->full_decisions.ptr : nullptr;
+        fa.carry = static_cast<unsigned long long *>(b->full_carry.ptr);
+        fa.claim = static_cast<int *>(b->full_claim.ptr);
+        fa.failed = static_cast<int *>(b->full_claim.ptr) + 1;
+        launch_full_sweep(fa, ctx->stream);
+        if (record) launch_full_walk(fa, ctx->stream);
+        if (hipPeekAtLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        b->full_pairs += 1;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+    b->full_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - began).count();
+    return 0;
+}
+
 int64_t run(dinara_batch *b) {
     DeviceContext *ctx = b->ctx;
     std::lock_guard<std::mutex> serial(ctx->lock);
     DeviceGuard guard(ctx);
     (void)hipGetLastError();  // an earlier, unrelated call's error is not this run's
-    if (b->unit) return run_unit(b, true);
+    if (b->whole) {
+        // No band sweep: every slot unsettled, then the whole-matrix pass takes them all.
+        const int64_t slots = (int64_t)b->slot_pair.size();
+        b->launches = b->first_proven = 0;
+        b->run_ms = b->sweep_ms = b->walk_ms = 0;
+        if (slots > 0 && (hipMemsetAsync(b->status.ptr, 0xFF, slots * 4, ctx->stream) != hipSuccess ||
+                          (b->kind == DINARA_BATCH_ALIGNMENTS &&
+                           hipMemsetAsync(b->cigar_used.ptr, 0, 8, ctx->stream) != hipSuccess)))
+            return DINARA_OUT_OF_MEMORY;
+        const int64_t code = full_pass(b);
+        b->run_ms = b->full_ms;
+        b->ran = code == 0;
+        return code;
+    }
+    if (b->unit) {
+        if (const int64_t code = run_unit(b, true)) return code;
+        return full_pass(b);
+    }
     const int64_t slots = (int64_t)b->slot_pair.size();
     b->launches = b->first_proven = 0;
     b->run_ms = b->sweep_ms = b->walk_ms = 0;
@@ -1179,7 +1272,7 @@ int64_t run(dinara_batch *b) {
     (void)hipEventElapsedTime(&total, ctx->start, ctx->stop);
     b->run_ms = total;
     b->ran = true;
-    return 0;
+    return full_pass(b);
 }
 
 int64_t fetch(dinara_batch *b) {
@@ -1740,6 +1833,12 @@ int64_t dinara_batch_set_side_by_side(dinara_batch *b, int64_t on) {
     return 0;
 }
 
+int64_t dinara_batch_set_whole(dinara_batch *b, int64_t on) {
+    if (!b || b->free_ends || b->user_band) return DINARA_INVALID_MODE;
+    b->whole = on != 0;
+    return 0;
+}
+
 int64_t dinara_batch_set_persistent(dinara_batch *b, int64_t on) {
     if (!b) return DINARA_INVALID_MODE;
     b->persistent = on != 0;
@@ -1807,6 +1906,8 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 7: return b->graph_used;
         case 8: return b->trace_passes;
         case 9: return b->walked_first;
+        case 10: return b->full_pairs;
+        case 11: return b->full_ms;
         default: break;
     }
     if (what >= 600 && what < 1200 && what % 100 < EDIT_BINS) {

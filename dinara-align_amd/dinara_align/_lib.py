@@ -139,6 +139,8 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_set_side_by_side.restype = i64
     handle.dinara_batch_set_persistent.argtypes = [vp, i64]
     handle.dinara_batch_set_persistent.restype = i64
+    handle.dinara_batch_set_whole.argtypes = [vp, i64]
+    handle.dinara_batch_set_whole.restype = i64
     handle.dinara_batch_set_tuning.argtypes = [vp, i64, vp]
     handle.dinara_batch_set_tuning.restype = i64
     handle.dinara_batch_autotune.argtypes = [vp, vp]

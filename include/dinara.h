@@ -226,8 +226,9 @@ void dinara_batch_free(dinara_batch *batch);
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
  * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep); for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
  * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began, 1000+b and
- * 1100+b the word-columns (a pair's columns times its band's 64-bit words) its sweeps and traces covered. For
- * roofline reporting. No counterpart in the reference. */
+ * 1100+b the word-columns (a pair's columns times its band's 64-bit words) its sweeps and traces covered. Every
+ * batch: 10 the pairs no band settled that the whole-matrix path swept after the run, 11 that path's time (ms,
+ * host clock, included in no other figure). For roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
 
 /*
@@ -281,6 +282,11 @@ int64_t dinara_batch_set_keep_first(dinara_batch *batch, int64_t on);
 /* Lane groups refill from their bin's queue (1, the default) or sweep one pair each (0: a launch as wide as the
  * bin's pairs, for comparison). */
 int64_t dinara_batch_set_persistent(dinara_batch *batch, int64_t on);
+/* Every pair of the batch (any costs, global, no user band) takes the whole-matrix path that serves the pairs no
+ * band settles (1), or the band sweeps first (0, the default): for checking the one against the other.
+ * DINARA_INVALID_MODE for a batch under free ends or a user band. No counterpart in the reference (the whole
+ * matrix without a band is what its CPU path fills, lanes.mojo:795-947). */
+int64_t dinara_batch_set_whole(dinara_batch *batch, int64_t on);
 /* The launch shape: persistent workgroups a compute unit (1..64), and for each class of the pairs' longer side
  * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..8); null keeps them. */
 int64_t dinara_batch_set_tuning(dinara_batch *batch, int64_t grid_per_cu, const int64_t *start_bins);
