@@ -221,11 +221,13 @@ def run_learned(args) -> int:
     eval_n = args.pairs or 200_000
     cores = host_cores()
     workers = min(cores, 16)
+    held_n = max(eval_n // 2, 1)
     t0 = time.time()
     train_r, train_q = synthetic.uneven_parallel(train_n, seed=41, workers=workers)
     eval_r, eval_q = synthetic.uneven_parallel(eval_n, seed=42, workers=workers)
-    print(f"[bench] uneven workload ready: {train_n} training, {eval_n} evaluation pairs in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    held_r, held_q = synthetic.uneven_parallel(held_n, seed=43, workers=workers)
+    print(f"[bench] uneven workload ready: {train_n} training, {held_n} held-out, {eval_n} evaluation pairs in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     handle = _lib.lib()
     if handle.dinara_device_count() < 1:
         print("[bench] no GPU visible", file=sys.stderr)
@@ -293,8 +295,8 @@ def run_learned(args) -> int:
         first = handle.dinara_batch_stat(b, 4)
         return float(np.mean(walls) * 1e3), float(np.mean(dev)), cls / args.steps, launches, first
 
-    def path(doubling, margin):
-        b, keep = batch_of(eval_r, eval_q)
+    def path(doubling, margin, refs=eval_r, qrys=eval_q):
+        b, keep = batch_of(refs, qrys)
         handle.dinara_batch_set_retry(b, 1 if doubling else 0)
         seed_ms = 0.0
         if margin is not None:
@@ -303,19 +305,52 @@ def run_learned(args) -> int:
                 raise RuntimeError("seeding failed")
             seed_ms = (time.perf_counter() - t) * 1e3
         ms, dev, cls, launches, first = timed(b)
-        out = distances(b, eval_n)
+        out = distances(b, len(refs))
         handle.dinara_batch_free(b)
         del keep
         return {"ms": ms, "dev": dev, "cls": cls, "launches": launches, "first": first, "seed_ms": seed_ms,
                 "out": out}
 
+    # The seeds' margin is chosen on held-out pairs drawn like the workload, then frozen for the evaluation pairs;
+    # the certified path is timed on them too.
+    held = {margin: path(False, margin, held_r, held_q) for margin in LEARNED_MARGINS}
+    held_certified = path(False, None, held_r, held_q)
+    del held_r, held_q
+    best_margin = min(held, key=lambda k: held[k]["ms"] + held[k]["seed_ms"])
+    print("[bench] held-out: " + ", ".join(f"margin {k}: {v['ms'] + v['seed_ms']:.1f} ms" for k, v in held.items()) +
+          f", certified {held_certified['ms']:.1f} ms", file=sys.stderr, flush=True)
     doubling = path(True, None)
     certified = path(False, None)
-    tried = {margin: path(False, margin) for margin in LEARNED_MARGINS}
+    learned = path(False, best_margin)
     handle.dinara_mlp_free(net)
-    best_margin = min(tried, key=lambda k: tried[k]["ms"] + tried[k]["seed_ms"])
-    learned = tried[best_margin]
-    exact = float(np.mean([np.array_equal(doubling["out"], r["out"]) for r in [certified, *tried.values()]]))
+    exact = float(np.mean([np.array_equal(doubling["out"], r["out"]) for r in (certified, learned)]))
+    print(f"[bench] doubling {doubling['ms']:.1f} ms, certified {certified['ms']:.1f} ms, learned "
+          f"{learned['ms'] + learned['seed_ms']:.1f} ms (margin {best_margin})", file=sys.stderr, flush=True)
+
+    # The CPU baseline: the restated lanes path (oracle/baseline.cpp, 16-bit lanes, distances) on a sample of the
+    # evaluation pairs, those whose dearest path its 16-bit cells hold (lanes.mojo:317-331).
+    cpu = None
+    if not args.no_cpu:
+        lib, flags = load_baseline(cores)
+        rng = np.random.default_rng(5)
+        held_cells = lambda i: 2 * OPEN + (len(eval_r[i]) + len(eval_q[i])) * EXTEND + max(MISMATCH, OPEN + EXTEND)
+        pool = [int(i) for i in rng.permutation(eval_n) if held_cells(int(i)) < 16000]
+        sample = pool[:min(len(pool), args.cpu_sample or 20_000)]
+        Rs, Qs = _lib.Texts([eval_r[i] for i in sample]), _lib.Texts([eval_q[i] for i in sample])
+        cpu_costs = (ctypes.c_int64 * len(sample))()
+        t0 = time.perf_counter()
+        lib.baseline_alignments(len(sample), ctypes.cast(Rs.pointers, ctypes.c_void_p),
+                                ctypes.cast(Rs.lengths, ctypes.c_void_p), ctypes.cast(Qs.pointers, ctypes.c_void_p),
+                                ctypes.cast(Qs.lengths, ctypes.c_void_p), MISMATCH, OPEN, EXTEND, cores, cpu_costs,
+                                None, 0)
+        cpu_s = time.perf_counter() - t0
+        same = sum(int(cpu_costs[j] == learned["out"][i]) for j, i in enumerate(sample))
+        cpu = {"value": round(len(sample) / cpu_s, 1), "unit": "pairs/s", "cores": cores, "kind": "port",
+               "sample": f"{len(sample)} evaluation pairs drawn uniformly from the {len(pool)} whose dearest path "
+                         f"the lanes path's 16-bit cells hold (of {eval_n}), distances, 16-bit lanes x 32 pairs, "
+                         f"{flags}, {cpu_model()}", "equal_to_gpu": round(same / max(len(sample), 1), 4)}
+        print(f"[bench] CPU baseline {len(sample) / cpu_s:.0f} pairs/s on {cores} cores; equal on "
+              f"{same}/{len(sample)}", file=sys.stderr, flush=True)
     n = np.fromiter((len(r) for r in eval_r), dtype=np.int64, count=eval_n)
     m = np.fromiter((len(q) for q in eval_q), dtype=np.int64, count=eval_n)
     cells = proven_band_cells(n, m, np.maximum(learned["out"], 0), strict=False)
@@ -344,8 +379,8 @@ def run_learned(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"gap-affine (4,6,2) distances, {eval_n} pairs log-normal around 1.5 kbp (sigma 1, "
                                f"100 bp..20 kbp), 1-15% error (50/25/25); network trained on {train_n} pairs "
-                               "labelled by the aligner", "pairs": eval_n, "train_pairs": train_n,
-                   "epochs": epochs, "margin": best_margin},
+                               "labelled by the aligner, its guesses the first bands", "pairs": eval_n,
+                   "train_pairs": train_n, "epochs": epochs, "margin": best_margin},
         "plain_doubling": {"value": round(eval_n / (doubling["ms"] / 1e3), 1), "ms_per_step": round(doubling["ms"], 3),
                            "first_band_proved": round(doubling["first"] / eval_n, 4)},
         "plain_certified": {"value": round(eval_n / (certified["ms"] / 1e3), 1),
@@ -353,14 +388,16 @@ def run_learned(args) -> int:
                             "first_band_proved": round(certified["first"] / eval_n, 4)},
         "learned": {"ms_per_step": round(learned["ms"], 3), "seed_ms": round(learned["seed_ms"], 3),
                     "first_band_proved": round(learned["first"] / eval_n, 4),
-                    "by_margin_ms": {str(k): round(v["ms"] + v["seed_ms"], 3) for k, v in tried.items()}},
+                    "margin_chosen_on": f"{held_n} held-out pairs drawn like the workload (seed 43)",
+                    "by_margin_ms_held_out": {str(k): round(v["ms"] + v["seed_ms"], 3) for k, v in held.items()},
+                    "certified_ms_held_out": round(held_certified["ms"], 3)},
         "speedup_vs_plain_doubling": round(doubling["ms"] / learned_total, 3),
         "speedup_vs_plain_certified": round(certified["ms"] / learned_total, 3),
         "gcups": round(float((n * m).sum()) / (learned_total / 1e3) / 1e9, 2),
         "bit_exact_vs_plain": exact,
         "training": {"label_s": round(label_s, 2), "train_s": round(train_s, 2), "final_loss": round(loss.value, 5)},
         "roofline": roofline,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
         "kernels_ms_per_step": {name: {CLASS_NAMES[c]: round(float(r["cls"][c]), 3) for c in range(CLASSES)
                                        if r["cls"][c] > 0}
                                 for name, r in (("doubling", doubling), ("certified", certified),
@@ -370,7 +407,116 @@ def run_learned(args) -> int:
     return 0
 
 
-LEARNED_MARGINS = (-0.05, 0.0, 0.05, 0.1)  # seeds e^(prediction + margin) - 1: room, or not, for the network's error
+# Seeds e^(prediction + margin) - 1: room, or not, for the network's error. Below zero, a first band that fails
+# still finds a cost near the pair's, whose certified band is then about the narrowest that proves it.
+LEARNED_MARGINS = (-0.6, -0.4, -0.2, 0.0, 0.2)
+
+FULL_OPS_PER_CELL = 14  # a whole-matrix cell: 3 adds and 5 mins of the recurrence, 6 ops of its decision nibble
+
+
+def run_oversized(args) -> int:
+    """Gap-affine (4, 6, 2) CIGARs of pairs of 70 kbp at about 1% edits: 140,000 letters a pair, past what every band
+    kernel stages in LDS, so each takes the whole-matrix path (full.hip) after the band sweeps leave it. The CPU
+    baseline is the oracle's proven-band trace (oracle.c, one core) of one of the pairs."""
+    count = args.pairs or 4
+    rng = np.random.default_rng(70)
+    letters = np.frombuffer(b"ACGT", dtype=np.uint8)
+    refs, qrys = [], []
+    for _ in range(count):
+        r = letters[rng.integers(0, 4, 70_000)]
+        q = r.copy()
+        at = rng.choice(70_000, 700, replace=False)
+        q[at] = letters[rng.integers(0, 4, 700)]  # about 0.75% substitutions
+        cut = np.sort(rng.choice(69_000, 20, replace=False))
+        pieces, last = [], 0
+        for i, c in enumerate(cut):  # ten deletions and ten insertions of 1-8 letters
+            pieces.append(q[last:c])
+            if i % 2:
+                pieces.append(letters[rng.integers(0, 4, int(rng.integers(1, 9)))])
+                last = c
+            else:
+                last = c + int(rng.integers(1, 9))
+        pieces.append(q[last:])
+        refs.append(r.tobytes())
+        qrys.append(np.concatenate(pieces).tobytes())
+    handle = _lib.lib()
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    costs = _lib.CCosts(MISMATCH, OPEN, EXTEND, -1, 0, 0, 0, -1, 0)
+    status = ctypes.c_int64(0)
+    b = handle.dinara_batch_new(0, _lib.BATCH_ALIGNMENTS, count, *R.args(), *Q.args(), ctypes.byref(costs), None,
+                                None, 16, ctypes.byref(status))
+    if not b:
+        raise RuntimeError(f"dinara_batch_new failed: {status.value}")
+    for _ in range(args.warmup):
+        if handle.dinara_batch_run(b) != 0:
+            raise RuntimeError("warm-up failed")
+    walls, whole = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        if handle.dinara_batch_run(b) != 0:
+            raise RuntimeError("run failed")
+        walls.append((time.perf_counter() - t) * 1e3)
+        whole.append(handle.dinara_batch_stat(b, 11))
+    swept = handle.dinara_batch_stat(b, 10)
+    found = (_lib.CAlignment * count)()
+    statuses = (ctypes.c_int64 * count)()
+    if handle.dinara_batch_alignments(b, found, statuses) != 0:
+        raise RuntimeError("alignments failed")
+    got = []
+    for i in range(count):
+        got.append((found[i].cost, ctypes.string_at(found[i].cigar, found[i].cigar_length)) if statuses[i] == 0
+                   else (int(statuses[i]), b""))
+        if statuses[i] == 0:
+            handle.dinara_free(found[i].cigar)
+    handle.dinara_batch_free(b)
+    ms = float(np.mean(walls))
+    whole_ms = float(np.mean(whole))
+    cells = float(sum(len(r) * len(q) for r, q in zip(refs, qrys)))
+    achieved = cells * FULL_OPS_PER_CELL / (whole_ms / 1e3) / 1e12
+    print(f"[bench] {count} pairs of 70 kbp: {ms:.1f} ms a run, whole-matrix pass {whole_ms:.1f} ms over {swept:.0f} "
+          f"pairs", file=sys.stderr, flush=True)
+    cpu = None
+    exact = None
+    if not args.no_cpu:
+        from oracle import oracle  # the checker, on the CPU leg only
+
+        oc = oracle.Costs.affine(MISMATCH, OPEN, EXTEND)
+        t0 = time.perf_counter()
+        want = oracle.costs_align_proven(refs[0], qrys[0], oc)
+        cpu_s = time.perf_counter() - t0
+        exact = bool(want[0] == got[0][0] and want[1].encode() == got[0][1])
+        cpu = {"value": round(1 / cpu_s, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": "one of the pairs: the oracle's trace of its strict proven band (oracle.c "
+                         f"orc_costs_align_proven), {cpu_s:.1f} s", "equal_to_gpu": exact}
+        print(f"[bench] CPU (oracle, proven band) {cpu_s:.1f} s for one pair; equal: {exact}", file=sys.stderr,
+              flush=True)
+    line = {
+        "metric": "pairs/s (gap-affine CIGARs of 70 kbp pairs past every band kernel: the whole-matrix path)",
+        "value": round(count / (ms / 1e3), 3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{count} pairs of 70,000 x about 70,000 letters, about 1% edits, gap-affine (4,6,2) "
+                               "CIGARs, Ties.LEFT", "pairs": count},
+        "whole_matrix_ms": round(whole_ms, 3),
+        "pairs_on_the_whole_matrix_path": swept,
+        "gcups": round(cells / (whole_ms / 1e3) / 1e9, 2),
+        "roofline": {"bound": "valu", "kernel": "full_sweep<1> + full_walk", "achieved": round(achieved, 3),
+                     "peak": VALU_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "traffic": None, "launch_ms": round(whole_ms / count, 3),
+                     "note": f"every cell of the matrix x {FULL_OPS_PER_CELL} int32 ops, over the whole-matrix pass "
+                             "(sweep, walk and CIGAR text of each pair, host clock)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
 
 
 LOCAL_OPS_PER_CELL = 10  # a local cell: 4 adds, 5 maxes and the floor at zero, the counting convention
@@ -1184,12 +1330,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--pairs", type=int, default=None, help="pairs (default: the workload's own count)")
-    ap.add_argument("--workload", choices=("uneven", "affine", "local", "learned", "edit"), default="uneven",
+    ap.add_argument("--workload", choices=("uneven", "affine", "local", "learned", "edit", "oversized"),
+                    default="uneven",
                     help="uneven: unit-cost CIGARs of 200k uneven pairs, learned band + persistent launches against "
                          "plain band doubling (the default); affine: global gap-affine (4,6,2) CIGARs, 100k x 250 bp; "
                          "local: local alignment under a 20-letter table, 50k x (3000 aa window, 300 aa query); "
                          "learned: gap-affine distances of 200k uneven pairs, the learned band against plain band "
-                         "doubling; edit: unit-cost distances, 10k x 150 bp")
+                         "doubling; edit: unit-cost distances, 10k x 150 bp; oversized: gap-affine CIGARs of 70 kbp "
+                         "pairs, past every band kernel's letters (the whole-matrix path)")
     ap.add_argument("--train", type=int, default=None, help="learned: training pairs (default 1,000,000)")
     ap.add_argument("--epochs", type=int, default=None, help="learned: training epochs (default 3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -1229,6 +1377,8 @@ def main() -> int:
         return run_uneven(args)
     if args.workload == "edit":
         return run_edit(args)
+    if args.workload == "oversized":
+        return run_oversized(args)
     args.pairs = args.pairs or 100_000
 
     t0 = time.time()

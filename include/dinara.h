@@ -15,6 +15,9 @@
  * prefix and suffix presets, costs minimized, no user band); and, under a substitution table, global and local
  * scores and alignments. Other modes (extension, local or free ends under Costs with a match score, free ends
  * under a user band) return DINARA_INVALID_MODE for the whole batch: this library holds no CPU fallback.
+ * A global pair no band settles (its band wider than the widest, or its letters more than the band kernels
+ * stage) is swept whole after the banded run; it reports DINARA_OUT_OF_MEMORY only when its decisions would
+ * pass 16 GiB, or under free ends or a user band.
  */
 #ifndef DINARA_H
 #define DINARA_H

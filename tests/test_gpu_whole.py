@@ -181,16 +181,16 @@ def test_pairs_no_band_settles():
 
 
 def test_pairs_past_the_band_kernels_letters():
-    # 68,000 letters a pair, past the 64 KiB the gap-affine band kernels stage: close pairs, whose proven band the
-    # oracle traces in little memory.
+    # 68,000 and 140,000 letters a pair (a 70 kbp pair, as the reference's device tests align), past the 64 KiB the
+    # gap-affine band kernels stage: close pairs, whose proven band the oracle traces in little memory.
     rng = random.Random(10)
     costs = da.Costs.affine(4, 6, 2)
     oc = to_oracle(costs)
     refs, qrys = [], []
-    for _ in range(2):
-        r = "".join(rng.choice("ACGT") for _ in range(34_000))
+    for length, edits in ((34_000, 300), (70_000, 250)):
+        r = "".join(rng.choice("ACGT") for _ in range(length))
         q = list(r)
-        for _ in range(300):
+        for _ in range(edits):
             at = rng.randint(0, len(q) - 1)
             kind = rng.random()
             if kind < 0.4:
