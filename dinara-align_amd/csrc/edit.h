@@ -72,9 +72,10 @@ struct EditArgs {
 };
 
 // The bytes a pair of `cols` query letters traced in bin `bin` keeps: its columns eight at a time, each word's eight
-// columns 128 contiguous bytes (two 64-bit planes a column), the pair's words side by side.
+// columns 128 contiguous bytes (two 64-bit planes a column), the pair's words side by side; one line more, for a
+// run of lines that starts its columns at a place past the first.
 __host__ __device__ inline unsigned long long edit_trace_bytes(int cols, int bin) {
-    return (unsigned long long)((cols + 7) / 8) * edit_lanes(bin) * edit_words(bin) * 128ull;
+    return (unsigned long long)((cols + 7) / 8 + 1) * edit_lanes(bin) * edit_words(bin) * 128ull;
 }
 
 // Pairs' longer sides split at 256, 1024, 4096 and 16384 letters: each class has its own least first bin, the

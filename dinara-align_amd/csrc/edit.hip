@@ -189,7 +189,15 @@ constexpr int SWEEP = 0, TRACE = 1, SWEEP_TRACED = 2;
 // Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. Kept decisions are 2 bits a cell (0
 // two unequal letters, 1 a reference letter alone, 2 a query letter alone, 3 two equal letters: the walk's first
 // true choice in that order) in two bit-planes: word w's columns 8q..8q+7 in the 128 bytes at arena + offset +
-// (q words + w) 128, 16 a column. Each lane holds its last four columns and writes their 64 bytes at once.
+// (q words + w) 128, 16 a column.
+//
+// A lane of one word (K = 1) writes whole lines in step with its wave: a pair claimed when its wave has made t0
+// column steps starts its columns at place `phase` = t0 mod 8 of its first line (its column j at place phase + j of
+// the line run), so every lane of the wave fills a line at the same step (t mod 8 = 7). Each column goes to LDS; at
+// those steps the wave writes the lanes' lines together, eight lanes a line, eight whole lines a store instruction,
+// and a pair ending mid-line writes its last line itself. The phase rides in the low bits of the pair's decision
+// offset (which is a multiple of 128). Lanes of more words hold their last four columns in registers and write
+// their 64 bytes at once, phase 0.
 template <int L, int K, int BITS, int MODE>
 __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     constexpr bool TRACE = MODE == 1;          // sweeps settled pairs' traced bands again
@@ -223,9 +231,15 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     bool recording = false;    // this group keeps the current pair's decisions
     long long trace_at = 0;    // and where, from the arena's start
     constexpr int WL = L * K;  // the pair's words
-    // The last HOLD columns' decision planes, oldest first: eight (a whole 128-byte line a word) when a lane holds one
-    // word, four (half a line) when it holds more.
-    constexpr int HOLD = K == 1 ? 8 : 4;
+    // One word a lane: columns staged in LDS, eight places a lane, a place row padded by one lane so the wave's
+    // line reads (eight lanes a line) fall in different banks.
+    constexpr bool STAGED = RECORD && K == 1;
+    __shared__ ulonglong2 stage[STAGED ? 4 : 1][STAGED ? 8 : 1][STAGED ? 65 : 1];
+    ulonglong2 (*const places)[STAGED ? 65 : 1] = stage[STAGED ? threadIdx.x >> 6 : 0];
+    int t = 0;       // the wave's column steps so far (the same in every lane)
+    int t0 = 0;      // the step count when the lane's pair was claimed
+    // More words a lane: the last HOLD columns' decision planes in registers, oldest first.
+    constexpr int HOLD = STAGED ? 1 : 4;
     uint64_t held0[HOLD][K], held1[HOLD][K];
 #pragma unroll
     for (int q = 0; q < HOLD; ++q)
@@ -281,9 +295,12 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     }
                     bottom.begin(ref, -lo);
                     query.begin(qry, 0);
+                    t0 = t;
                     if (TRACE) {
-                        trace_at = a.trace_offset[slot];
+                        trace_at = a.trace_offset[slot] & ~127ll;
                         recording = true;
+                        // The walk reads where this pass puts the pair's columns.
+                        if (first_lane) a.trace_offset[slot] = trace_at | (STAGED ? (t0 & 7) : 0);
                     } else if (RECORD) {
                         // Room for this band's decisions, if the arena has it; else the pair sweeps without.
                         long long at = -1;
@@ -345,7 +362,14 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 #pragma unroll
             for (int d = L / 2; d >= 1; d /= 2) part += __shfl_xor(part, d);
             const int found = hi + m + top + part;  // `top` is the first lane's
-            if constexpr (RECORD) {
+            if constexpr (STAGED) {
+                // A pair ending before its line's last place: that line from LDS, by its own lane.
+                const int last = (t - 1) & 7;  // the place of its last column
+                if (finishing && recording && m > 0 && last != 7) {
+                    uint8_t *line = trace + ((size_t)(((t - 1) >> 3) - (t0 >> 3)) * WL + k) * 128;
+                    for (int q = 0; q <= last; ++q) reinterpret_cast<ulonglong2 *>(line)[q] = places[q][lane];
+                }
+            } else if constexpr (RECORD) {
                 // The last, partly filled columns: shifted down to the chunk's start, then written whole.
                 if (finishing && recording && (m % HOLD)) {
                     for (int pad = m % HOLD; pad < HOLD; ++pad) {
@@ -381,7 +405,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                         a.s.status[slot] = 0;
                         a.s.walk_low[slot] = lo;
                         a.s.walk_class[slot] = bin;
-                        a.trace_offset[slot] = trace_at;
+                        a.trace_offset[slot] = trace_at | (STAGED ? (t0 & 7) : 0);
                         a.walk_ready[slot] = 1;
                         atomicAdd(&a.stats[2 * EDIT_BINS + 1], 1u);
                     } else {
@@ -411,6 +435,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
         // One column. The window moves down a row: each word takes the bit of the word below; the band's bottom
         // takes +1 under it (Pv), and the new bottom row's letter.
         ++j;
+        const int place = t & 7;  // this column's place in its line (every lane alike)
+        ++t;
         if constexpr (BITS == 2) {
             // Each plane slides down a row: its top word's bit 63 comes from bit 0 of the next lane's lowest word (one
             // DPP move a plane), or at the band's bottom from the stand-ins (+1 vertical, no -1) and the new bottom
@@ -495,7 +521,32 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             const uint64_t mhs = (mh[kk] << 1) | (kk ? mh[kk > 0 ? kk - 1 : 0] >> 63 : (uint64_t)((carried >> 1) & 1));
             const uint64_t npv = mhs | ~(xv[kk] | phs);
             const uint64_t nmv = phs & xv[kk];
-            if constexpr (RECORD) {
+            if constexpr (STAGED) {
+                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
+                const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
+                places[place][lane] = make_ulonglong2(~unequal_step & (npv | ~ph[kk]), ~unequal_step & ~npv);
+                if (place == 7) {
+                    // Every lane's line is full: the wave writes them, lanes 8g .. 8g + 7 the eight places of lane
+                    // g' = 8i + g's line in instruction i.
+                    const unsigned long long writers = __ballot(active && recording);
+                    if (writers) {
+                        __builtin_amdgcn_wave_barrier();
+                        const uint64_t mine = (uint64_t)(uintptr_t)(trace + ((size_t)(((t - 1) >> 3) - (t0 >> 3)) * WL + k) * 128);
+                        const int piece = lane & 7;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            if (((writers >> (8 * i)) & 0xFFull) == 0) continue;
+                            const int from = 8 * i + (lane >> 3);
+                            const uint32_t at_lo = __shfl((uint32_t)mine, from);
+                            const uint32_t at_hi = __shfl((uint32_t)(mine >> 32), from);
+                            const ulonglong2 v = places[piece][from];
+                            if ((writers >> from) & 1ull)
+                                reinterpret_cast<ulonglong2 *>(((uint64_t)at_hi << 32) | at_lo)[piece] = v;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            } else if constexpr (RECORD) {
                 // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
                 const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
 #pragma unroll
@@ -584,6 +635,7 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
     }
     OpTape ops{};
     int r = 0, c = 0, i = 0, width = 0, words = 0;
+    int phase = 0;  // the place of the pair's column 0 in its first line
     const uint8_t *base = a.arena;
     uint32_t unequal_marks = 0;  // without eqx two unequal letters are 'M' like two equal ones
     bool outside = false;
@@ -594,7 +646,11 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
         width = edit_width(bin);
         words = edit_lanes(bin) * edit_words(bin);
         i = a.s.walk_low[slot] + width - 1 - c + r;  // bit of the band's window: row r's diagonal c - r
-        if (r > 0 && c > 0) base += a.trace_offset[slot];
+        if (r > 0 && c > 0) {
+            const int64_t at = a.trace_offset[slot];
+            base += at & ~int64_t(127);
+            phase = (int)(at & 7);
+        }
         ops = op_tape(a.s, slot);
         unequal_marks = a.s.eqx ? 0x55555555u : 0u;
     }
@@ -615,7 +671,8 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
     int half = 0, part = 0;           // hold the lane's diagonal (`half`), in the span's `part` (0 low, 1 high)
     int now_tag = -1, loaded_tag = -1;
     int need = 0;  // the span the lane walks next
-    const int chunks = (c + 7) >> 3;
+    // Columns are read at their places: column c - 1 (from 0) is place c - 1 + phase of the pair's line run.
+    const int chunks = (c + phase + 7) >> 3;
     const int lane = threadIdx.x;
     const uint64_t blank = (uint64_t)(uintptr_t)walk_blank;
     // Every lane of the wave: publish the span's four chunk addresses, then load piece `lane % 32` of the spans of
@@ -662,25 +719,27 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
         __builtin_amdgcn_wave_barrier();
         now_tag = loaded_tag;
     };
-    if (live) need = ((c - 1) >> 5) * words + (i >> 6);
+    if (live) need = ((c - 1 + phase) >> 5) * words + (i >> 6);
     issue(live ? need : -1);
     while (__any(live)) {
         hand_over();
         const bool walking = live && now_tag == need;
         const int expected = walking ? need - words : need;
         issue(live && expected >= 0 ? expected : -1);
-        read_half((i & 63) >> 5, ((c - 1) >> 4) & 1);
+        read_half((i & 63) >> 5, ((c - 1 + phase) >> 4) & 1);
         uint32_t choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
         if (walking) {
-            const int span = (c - 1) >> 5, word = i >> 6;
+            const int span = (c - 1 + phase) >> 5, word = i >> 6;
             for (;;) {
                 // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the half span's
                 // edge, then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
-                const int q = (c - 1) & 15;
+                const int q = (c - 1 + phase) & 15;
                 const uint32_t upto = q == 15 ? 0xFFFFFFFFu : (4u << (2 * q)) - 1u;
                 const uint32_t gaps = (choices ^ (choices >> 1)) & 0x55555555u & upto;
                 const int stop = gaps ? (31 - __builtin_clz(gaps)) >> 1 : -1;
-                const int steps = q - stop < r ? q - stop : r;
+                // No further than the rows left, nor than the columns left (the places before the first are none).
+                const int run = q - stop < r ? q - stop : r;
+                const int steps = run < c ? run : c;
                 const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
                 const uint32_t walked = (__builtin_bitreverse32(unequal) >> 1) >> (2 * (15 - q));
                 uint64_t codes = steps >= 16 ? (uint64_t)walked : (uint64_t)(walked & ((1u << (2 * steps)) - 1u));
@@ -696,7 +755,7 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
                 i += insertion - deletion;
                 const bool finished = r == 0 || c == 0;
                 const bool astray = !finished && (i < 0 || i >= width);
-                if (finished || astray || (i >> 6) != word || ((c - 1) >> 5) != span) {
+                if (finished || astray || (i >> 6) != word || ((c - 1 + phase) >> 5) != span) {
                     if (astray) {  // a proven band holds every step; this pair's did not
                         a.s.status[slot] = 3;
                         outside = true;
@@ -705,10 +764,11 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
                     break;
                 }
                 // Into the span's low half, or the diagonal across bit 31/32: those columns' halves from LDS.
-                if (((i & 63) >> 5) != half || (((c - 1) >> 4) & 1) != part) read_half((i & 63) >> 5, ((c - 1) >> 4) & 1);
+                if (((i & 63) >> 5) != half || (((c - 1 + phase) >> 4) & 1) != part)
+                    read_half((i & 63) >> 5, ((c - 1 + phase) >> 4) & 1);
                 choices = diagonal_choices(half_x, half_y, (uint32_t)i & 31u);
             }
-            if (live) need = ((c - 1) >> 5) * words + (i >> 6);
+            if (live) need = ((c - 1 + phase) >> 5) * words + (i >> 6);
         }
     }
     hand_over();  // the last loads: every load the wave made is waited for before it ends
