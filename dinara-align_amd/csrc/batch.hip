@@ -264,7 +264,8 @@ struct dinara_batch {
     int may_need_full = -1;
     bool whole = false;  // every pair takes the whole-matrix path, no band sweep (for checking and comparison)
     double full_pairs = 0, full_ms = 0;
-    DeviceBuffer full_list, full_count, full_decisions, full_carry, full_claim;
+    DeviceBuffer full_list, full_count, full_decisions, full_carry, full_claim, full_slots, full_strips,
+        full_decision_at, full_carry_at;
 };
 
 namespace {
@@ -1177,32 +1178,61 @@ int64_t full_pass(dinara_batch *b) {
     std::sort(list.begin(), list.end());
     const bool record = b->kind == DINARA_BATCH_ALIGNMENTS;
     const int pieces = b->costs.pieces == 2 ? 2 : 1;
-    FullArgs fa{};
-    fa.s = sweep_args(b);
-    fa.record = record ? 1 : 0;
-    for (const int32_t slot : list) {
-        const int64_t n = b->rows[slot], m = b->cols[slot];
-        if (slot < 0 || slot >= slots) continue;
-        const int64_t strips = std::max<int64_t>(1, (m + FULL_STRIP - 1) / FULL_STRIP);
-        const size_t decisions = record ? (size_t)strips * n * 64 * (pieces == 2 ? 8 : 4) : 0;
-        if (decisions > FULL_BUDGET) continue;
-        const size_t carry = (size_t)strips * (n + 1) * 8 * pieces;
-        if ((record && !b->full_decisions.reserve(decisions)) || !b->full_carry.reserve(carry) ||
-            hipMemsetAsync(b->full_carry.ptr, 0, carry, ctx->stream) != hipSuccess ||
-            hipMemsetAsync(b->full_claim.ptr, 0, 8, ctx->stream) != hipSuccess)
+    const size_t record_bytes = pieces == 2 ? 8 : 4;
+    // The listed pairs in launches of as many as fit FULL_BUDGET together (a pair past it alone stays unsettled).
+    size_t next = 0;
+    bool launched = false;
+    while (next < list.size()) {
+        // The buffers below are refilled (and may move) for each launch: the one before must be done with them.
+        if (launched && hipStreamSynchronize(ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+        std::vector<int32_t> pair_slot, strip_first;
+        std::vector<int64_t> decision_at, carry_at;
+        int64_t strips = 0, records = 0, carry_words = 0;
+        for (; next < list.size(); ++next) {
+            const int32_t slot = list[next];
+            if (slot < 0 || slot >= slots) continue;
+            const int64_t n = b->rows[slot], m = b->cols[slot];
+            const int64_t own = std::max<int64_t>(1, (m + FULL_STRIP - 1) / FULL_STRIP);
+            const int64_t own_records = record ? own * n * 64 : 0;
+            if ((size_t)own_records * record_bytes > FULL_BUDGET) continue;
+            if (!pair_slot.empty() && (size_t)(records + own_records) * record_bytes > FULL_BUDGET) break;
+            pair_slot.push_back(slot);
+            strip_first.push_back((int32_t)strips);
+            decision_at.push_back(records);
+            carry_at.push_back(carry_words);
+            strips += own;
+            records += own_records;
+            carry_words += own * (n + 1) * pieces;
+        }
+        if (pair_slot.empty()) continue;
+        strip_first.push_back((int32_t)strips);
+        const int64_t count = (int64_t)pair_slot.size();
+        if ((record && !b->full_decisions.reserve((size_t)records * record_bytes)) ||
+            !b->full_carry.reserve((size_t)carry_words * 8) || !b->full_claim.reserve((size_t)(count + 1) * 4) ||
+            !b->full_slots.upload(ctx, pair_slot.data(), (size_t)count * 4) ||
+            !b->full_strips.upload(ctx, strip_first.data(), (size_t)(count + 1) * 4) ||
+            !b->full_decision_at.upload(ctx, decision_at.data(), (size_t)count * 8) ||
+            !b->full_carry_at.upload(ctx, carry_at.data(), (size_t)count * 8) ||
+            hipMemsetAsync(b->full_carry.ptr, 0, (size_t)carry_words * 8, ctx->stream) != hipSuccess ||
+            hipMemsetAsync(b->full_claim.ptr, 0, (size_t)(count + 1) * 4, ctx->stream) != hipSuccess)
             return DINARA_OUT_OF_MEMORY;
-        fa.slot = slot;
-        fa.rows = (int)n;
-        fa.cols = (int)m;
-        fa.strips = (int)strips;
+        FullArgs fa{};
+        fa.s = sweep_args(b);
+        fa.pairs = (int)count;
+        fa.pair_slot = static_cast<const int32_t *>(b->full_slots.ptr);
+        fa.strip_first = static_cast<const int32_t *>(b->full_strips.ptr);
+        fa.decision_at = static_cast<const int64_t *>(b->full_decision_at.ptr);
+        fa.carry_at = static_cast<const int64_t *>(b->full_carry_at.ptr);
+        fa.record = record ? 1 : 0;
         fa.decisions = record ? b->full_decisions.ptr : nullptr;
         fa.carry = static_cast<unsigned long long *>(b->full_carry.ptr);
         fa.claim = static_cast<int *>(b->full_claim.ptr);
         fa.failed = static_cast<int *>(b->full_claim.ptr) + 1;
-        launch_full_sweep(fa, ctx->stream);
+        launch_full_sweep(fa, (int)strips, ctx->stream);
         if (record) launch_full_walk(fa, ctx->stream);
         if (hipPeekAtLastError() != hipSuccess) return DINARA_OUT_OF_MEMORY;
-        b->full_pairs += 1;
+        b->full_pairs += (double)count;
+        launched = true;
     }
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
     b->full_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - began).count();

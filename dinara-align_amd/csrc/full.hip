@@ -11,8 +11,9 @@
 // a relaxed agent-scope atomic store: the record's own nonzero value says it is there, so no fence or flag is
 // needed, and agent scope keeps it coherent across the XCDs' L2s. The reading wave fetches 64 rows' records at
 // once (lane i row s + i) every 64 steps, and lane 0 takes row s's with a readlane. Strips run as a pipeline, each
-// about 130 rows behind the one to its left; waves claim strips in order from a counter, so the strip a wave
-// waits on is always held by a running wave, and every wait gives up after FULL_POLL_LIMIT polls.
+// about 130 rows behind the one to its left. One launch takes every listed pair, their strips numbered pair after
+// pair; waves claim strips in order from a counter, so the strip a wave waits on is always held by a running wave,
+// and every wait gives up after FULL_POLL_LIMIT polls.
 //
 // The reference's letters pass through a ring of four 256-letter chunks in LDS, as the Scoring sweep's long
 // references do; the lane's eight query letters sit in registers.
@@ -21,7 +22,7 @@
 // 0-1 the source: 0 two equal letters, 3 two unequal, 1 the deletion layer, 2 the insertion layer; bit 2 the
 // deletion extends into the cell, bit 3 the insertion does), eight cells a 32-bit lane record; two pieces, a byte
 // a cell (bits 0-2 the source, 0 equal, 5 unequal, 1-4 the gap layers; bit 2 + L layer L extends), eight a 64-bit
-// record. Record (strip, r, lane) is at ((strip * rows) + r - 1) * 64 + lane.
+// record. A pair's record (strip, r, lane) is at its first record + ((strip * rows) + r - 1) * 64 + lane.
 
 #include "bands.h"
 #include "full.h"
@@ -61,34 +62,45 @@ __global__ __launch_bounds__(64) void full_sweep(FullArgs a) {
     const uint8_t *ring = reinterpret_cast<const uint8_t *>(ring_words);
     const int lane = threadIdx.x;
     const CostParams k = a.s.costs;
-    const int rows = a.rows, cols = a.cols;
-    const int group = a.slot >> 6, column = a.slot & 63;
-    const uint32_t *ref_words = a.s.tape.words + a.s.tape.group_base[group] + column;
-    const uint32_t *qry_words = ref_words + (size_t)a.s.tape.group_ref_words[group] * 64;
     const int bits = a.s.tape.bits, per = 32 / bits;
     const uint32_t mask = (1u << bits) - 1;
-    if (rows == 0 || cols == 0) {  // one gap of every letter; the walk makes it
-        if (blockIdx.x == 0 && lane == 0) {
-            a.s.cost[a.slot] = rows == 0 ? gap_of<P>(k.ins_open, k.ins_ext, k.ins_open2, k.ins_ext2, cols)
-                                         : gap_of<P>(k.del_open, k.del_ext, k.del_open2, k.del_ext2, rows);
-            a.s.status[a.slot] = 0;
-        }
-        return;
-    }
-    // Four reference letters from `at` on, a byte each (zero past the end).
-    auto ref4 = [&](int at) {
-        uint32_t w = 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            if (at + t < rows) w |= tape_letter(ref_words, bits, per, mask, at + t) << (8 * t);
-        return w;
-    };
+    const int total = a.strip_first[a.pairs];
 
     for (;;) {
-        int strip = 0;
-        if (lane == 0) strip = atomicAdd(a.claim, 1);
-        strip = __builtin_amdgcn_readfirstlane(__shfl(strip, 0));
-        if (strip >= a.strips) return;
+        int claimed = 0;
+        if (lane == 0) claimed = atomicAdd(a.claim, 1);
+        claimed = __builtin_amdgcn_readfirstlane(__shfl(claimed, 0));
+        if (claimed >= total) return;
+        // The claimed strip's pair: the last whose first strip is at or before it.
+        int lo = 0, hi = a.pairs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.strip_first[mid] <= claimed) lo = mid; else hi = mid - 1;
+        }
+        const int pair = lo;
+        const int slot = a.pair_slot[pair];
+        const int strip = claimed - a.strip_first[pair];
+        const int strips = a.strip_first[pair + 1] - a.strip_first[pair];
+        const int rows = a.s.tape.rows[slot], cols = a.s.tape.cols[slot];
+        if (rows == 0 || cols == 0) {  // one gap of every letter; the walk makes it
+            if (lane == 0) {
+                a.s.cost[slot] = rows == 0 ? gap_of<P>(k.ins_open, k.ins_ext, k.ins_open2, k.ins_ext2, cols)
+                                           : gap_of<P>(k.del_open, k.del_ext, k.del_open2, k.del_ext2, rows);
+                a.s.status[slot] = 0;
+            }
+            continue;
+        }
+        const int group = slot >> 6, column = slot & 63;
+        const uint32_t *ref_words = a.s.tape.words + a.s.tape.group_base[group] + column;
+        const uint32_t *qry_words = ref_words + (size_t)a.s.tape.group_ref_words[group] * 64;
+        // Four reference letters from `at` on, a byte each (zero past the end).
+        auto ref4 = [&](int at) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (at + t < rows) w |= tape_letter(ref_words, bits, per, mask, at + t) << (8 * t);
+            return w;
+        };
         const int c0 = strip * FULL_STRIP + lane * FULL_C;  // the column left of the lane's first
         uint32_t q[FULL_C];
 #pragma unroll
@@ -104,9 +116,9 @@ __global__ __launch_bounds__(64) void full_sweep(FullArgs a) {
         }
         int left_prev = gap_of<P>(k.ins_open, k.ins_ext, k.ins_open2, k.ins_ext2, c0);  // H(r - 1, c0)
         int out_h = 0, out_i = INF, out_i2 = INF;  // the lane's last column, the row it just made
-        const unsigned long long *left_carry =
-            strip > 0 ? a.carry + (size_t)(strip - 1) * (rows + 1) * P : nullptr;
-        unsigned long long *own_carry = strip + 1 < a.strips ? a.carry + (size_t)strip * (rows + 1) * P : nullptr;
+        unsigned long long *carry = a.carry + a.carry_at[pair];
+        const unsigned long long *left_carry = strip > 0 ? carry + (size_t)(strip - 1) * (rows + 1) * P : nullptr;
+        unsigned long long *own_carry = strip + 1 < strips ? carry + (size_t)strip * (rows + 1) * P : nullptr;
         unsigned long long fetched = 0, fetched2 = 0;  // lane i: the left strip's record of row (s - 1 & ~63) + 1 + i
 
         // The ring's first two chunks, the third held in a register (the previous strip's reads are done).
@@ -224,7 +236,7 @@ __global__ __launch_bounds__(64) void full_sweep(FullArgs a) {
                 out_i = il;
                 out_i2 = il2;
                 left_prev = lh;
-                const size_t at = ((size_t)strip * rows + (r - 1)) * 64 + lane;
+                const size_t at = (size_t)a.decision_at[pair] + ((size_t)strip * rows + (r - 1)) * 64 + lane;
                 if (a.record) {
                     if (P == 1)
                         static_cast<uint32_t *>(a.decisions)[at] = record;
@@ -242,14 +254,14 @@ __global__ __launch_bounds__(64) void full_sweep(FullArgs a) {
 #pragma unroll
                     for (int i = 0; i < FULL_C; ++i)
                         if (c0 + i + 1 == cols) {
-                            a.s.cost[a.slot] = H[i];
-                            a.s.status[a.slot] = 0;
+                            a.s.cost[slot] = H[i];
+                            a.s.status[slot] = 0;
                         }
                 }
             }
         }
         if (gave_up) {
-            if (lane == 0) atomicExch(a.failed, 1);
+            if (lane == 0) atomicExch(a.failed + pair, 1);
             return;
         }
     }
@@ -263,10 +275,13 @@ template <int P>
 __global__ __launch_bounds__(64) void full_walk(FullArgs a) {
     const int lane = threadIdx.x;
     const SweepArgs &s = a.s;
-    const int rows = a.rows, cols = a.cols;
+    const int pair = blockIdx.x;
+    const int slot = a.pair_slot[pair];
+    const int rows = s.tape.rows[slot], cols = s.tape.cols[slot];
+    const int64_t first = a.decision_at[pair];
     int64_t moves = -1;
-    if (a.record && *a.failed == 0 && s.status[a.slot] == 0 && s.cost[a.slot] <= s.cap) {
-        OpTape ops = op_tape(s, a.slot);
+    if (a.record && a.failed[pair] == 0 && s.status[slot] == 0 && s.cost[slot] <= s.cap) {
+        OpTape ops = op_tape(s, slot);
         const uint32_t unequal = s.eqx ? 1u : 0u;
         int r = rows, c = cols, layer = 0;
         while (r > 0 && c > 0) {
@@ -276,7 +291,7 @@ __global__ __launch_bounds__(64) void full_walk(FullArgs a) {
             uint32_t flag = 0;
             if (lr > 0 && lc > 0) {
                 const int strip = (lc - 1) / FULL_STRIP, within = (lc - 1) % FULL_STRIP;
-                const size_t at = ((size_t)strip * rows + (lr - 1)) * 64 + within / FULL_C;
+                const size_t at = (size_t)first + ((size_t)strip * rows + (lr - 1)) * 64 + within / FULL_C;
                 const int cell = within % FULL_C;
                 if (P == 1)
                     flag = (static_cast<const uint32_t *>(a.decisions)[at] >> (4 * cell)) & 15u;
@@ -320,7 +335,7 @@ __global__ __launch_bounds__(64) void full_walk(FullArgs a) {
             moves = ops.count;
         }
     }
-    publish_cigars(s, a.slot, moves);
+    publish_cigars(s, slot, moves);
 }
 
 __global__ __launch_bounds__(256) void full_collect(const int32_t *status, int slots, int32_t *list, int32_t *count) {
@@ -330,8 +345,9 @@ __global__ __launch_bounds__(256) void full_collect(const int32_t *status, int s
 
 }  // namespace
 
-void launch_full_sweep(const FullArgs &a, hipStream_t stream) {
-    const int grid = a.strips < FULL_GRID ? a.strips : FULL_GRID;
+void launch_full_sweep(const FullArgs &a, int strips, hipStream_t stream) {
+    if (a.pairs <= 0) return;
+    const int grid = strips < FULL_GRID ? strips : FULL_GRID;
     if (a.s.costs.pieces == 2)
         hipLaunchKernelGGL(full_sweep<2>, dim3(grid), dim3(64), 0, stream, a);
     else
@@ -339,10 +355,11 @@ void launch_full_sweep(const FullArgs &a, hipStream_t stream) {
 }
 
 void launch_full_walk(const FullArgs &a, hipStream_t stream) {
+    if (a.pairs <= 0) return;
     if (a.s.costs.pieces == 2)
-        hipLaunchKernelGGL(full_walk<2>, dim3(1), dim3(64), 0, stream, a);
+        hipLaunchKernelGGL(full_walk<2>, dim3(a.pairs), dim3(64), 0, stream, a);
     else
-        hipLaunchKernelGGL(full_walk<1>, dim3(1), dim3(64), 0, stream, a);
+        hipLaunchKernelGGL(full_walk<1>, dim3(a.pairs), dim3(64), 0, stream, a);
 }
 
 void launch_full_collect(const int32_t *status, int slots, int32_t *list, int32_t *count, hipStream_t stream) {
