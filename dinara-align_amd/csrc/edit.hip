@@ -198,6 +198,13 @@ constexpr int SWEEP = 0, TRACE = 1, SWEEP_TRACED = 2;
 // and a pair ending mid-line writes its last line itself. The phase rides in the low bits of the pair's decision
 // offset (which is a multiple of 128). Lanes of more words hold their last four columns in registers and write
 // their 64 bytes at once, phase 0.
+// Sixteen bytes of kept decisions in global memory, stored through an address held as an integer (a global store,
+// not a flat one).
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef u64x2 __attribute__((address_space(1))) global_decisions;
+
+__device__ __forceinline__ u64x2 planes_of(ulonglong2 v) { return u64x2{v.x, v.y}; }
+
 template <int L, int K, int BITS, int MODE>
 __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     constexpr bool TRACE = MODE == 1;          // sweeps settled pairs' traced bands again
@@ -367,7 +374,9 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                 const int last = (t - 1) & 7;  // the place of its last column
                 if (finishing && recording && m > 0 && last != 7) {
                     uint8_t *line = trace + ((size_t)(((t - 1) >> 3) - (t0 >> 3)) * WL + k) * 128;
-                    for (int q = 0; q <= last; ++q) reinterpret_cast<ulonglong2 *>(line)[q] = places[q][lane];
+                    for (int q = 0; q <= last; ++q)
+                        reinterpret_cast<global_decisions *>((uint64_t)(uintptr_t)line)[q] =
+                            planes_of(places[q][lane]);
                 }
             } else if constexpr (RECORD) {
                 // The last, partly filled columns: shifted down to the chunk's start, then written whole.
@@ -384,9 +393,9 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     }
 #pragma unroll
                     for (int kk = 0; kk < K; ++kk) {
-                        ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + held_chunk_at<HOLD>(m / HOLD, WL, k * K + kk));
+                        global_decisions *chunk = reinterpret_cast<global_decisions *>((uint64_t)(uintptr_t)trace + held_chunk_at<HOLD>(m / HOLD, WL, k * K + kk));
 #pragma unroll
-                        for (int q = 0; q < HOLD; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
+                        for (int q = 0; q < HOLD; ++q) chunk[q] = u64x2{held0[q][kk], held1[q][kk]};
                     }
                 }
             }
@@ -541,7 +550,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                             const uint32_t at_hi = __shfl((uint32_t)(mine >> 32), from);
                             const ulonglong2 v = places[piece][from];
                             if ((writers >> from) & 1ull)
-                                reinterpret_cast<ulonglong2 *>(((uint64_t)at_hi << 32) | at_lo)[piece] = v;
+                                *reinterpret_cast<global_decisions *>((((uint64_t)at_hi << 32) | at_lo) +
+                                                                      (uint64_t)piece * 16) = planes_of(v);
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
@@ -557,9 +567,9 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                 held0[HOLD - 1][kk] = ~unequal_step & (npv | ~ph[kk]);
                 held1[HOLD - 1][kk] = ~unequal_step & ~npv;
                 if (active && recording && (j % HOLD) == 0) {
-                    ulonglong2 *chunk = reinterpret_cast<ulonglong2 *>(trace + held_chunk_at<HOLD>(j / HOLD - 1, WL, k * K + kk));
+                    global_decisions *chunk = reinterpret_cast<global_decisions *>((uint64_t)(uintptr_t)trace + held_chunk_at<HOLD>(j / HOLD - 1, WL, k * K + kk));
 #pragma unroll
-                    for (int q = 0; q < HOLD; ++q) chunk[q] = make_ulonglong2(held0[q][kk], held1[q][kk]);
+                    for (int q = 0; q < HOLD; ++q) chunk[q] = u64x2{held0[q][kk], held1[q][kk]};
                 }
             }
             pv[kk] = npv;
