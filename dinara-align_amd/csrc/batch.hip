@@ -344,7 +344,7 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
         words += (int64_t)(rw + qw) * 64;
     }
     // Every word is written below (a pair's words past its letters are zero), so the tape needs no clearing but
-    // its padding: the unit-cost sweeps' letter cursors read up to two words past a sequence's last without a bounds
+    // its padding: the unit-cost sweeps' letter cursors read up to three words past a sequence's last without a bounds
     // check (edit.hip), which past the last group's words lands here.
     const int64_t padded = words + TAPE_PADDING_WORDS;
     std::unique_ptr<uint32_t[]> tape(new (std::nothrow) uint32_t[padded]);

@@ -7,6 +7,8 @@
 // letter alone, else a query letter alone, and two equal letters only when no edit is; Ties.LEFT walks both
 // sequences reversed (edit_distance.mojo:295-297). At unit costs that rule and the reference's DP-flag rule
 // (lanes.mojo:905-923) give the same CIGAR.
+#include <type_traits>
+
 #include "bands.h"
 #include "edit.h"
 
@@ -78,13 +80,13 @@ struct Cursor {
     const uint32_t *text;
     uint32_t cur, ahead, left, next_word;
 
-    __device__ void park() {
+    __device__ void park(const uint32_t *) {
         text = nullptr;
         cur = ahead = 0;
         left = 0xFFFFFFFFu;  // never refills
         next_word = 0;
     }
-    __device__ void begin(const uint32_t *t, int from) {
+    __device__ void begin(const uint32_t *t, int from, int) {
         text = t;
         const uint32_t w = (uint32_t)from / PER, r = (uint32_t)from % PER;
         cur = text[(size_t)w * 64] >> (BITS * r);
@@ -103,7 +105,51 @@ struct Cursor {
         }
         return code;
     }
+    __device__ void refill() {}  // refills on its own, letter by letter
 };
+
+// One sequence of 2-bit letters read a letter a column, refilled only at the steps the whole wave shares (its
+// column steps t with t mod 8 = 7, the ends of its eight-column periods): `now` holds the letters up to the period's
+// end, low bits first; at the period's last column the next eight are cut from the two words loaded at the period's
+// start (`word0`, `word1`, the words that hold letters `ahead` .. `ahead` + 7), and the words for the eight after
+// are loaded. No lane branches on its own position and no load is used fewer than eight columns after it is made,
+// so a column never waits on memory (the stores the wave makes at a period's end included: a wait for a load also
+// waits for every store made before it). A refill reads at most one word past the one holding letter `ahead` + 7,
+// where `ahead` is at most one past the last letter a column used; a parked lane stays on the tape's first words.
+struct Letters2 {
+    const uint32_t *text;
+    uint32_t now, word0, word1, ahead, live;
+
+    __device__ void park(const uint32_t *tape) {
+        text = tape;
+        now = word0 = word1 = ahead = live = 0;
+    }
+    // Letters from `from` on, the first taken at a column whose place in its period is `place`.
+    __device__ void begin(const uint32_t *t, uint32_t from, int place) {
+        text = t;
+        const uint32_t w = from / 16;
+        now = __builtin_amdgcn_alignbit(text[(size_t)(w + 1) * 64], text[(size_t)w * 64], 2 * (from % 16));
+        ahead = from + (uint32_t)(8 - place);
+        live = 8;
+        word0 = text[(size_t)(ahead / 16) * 64];
+        word1 = text[(size_t)(ahead / 16 + 1) * 64];
+    }
+    __device__ uint32_t next() {
+        const uint32_t code = now & 3u;
+        now >>= 2;
+        return code;
+    }
+    // At the period's last column, after its letter is taken.
+    __device__ void refill() {
+        now = __builtin_amdgcn_alignbit(word1, word0, 2 * (ahead % 16));
+        ahead += live;
+        word0 = text[(size_t)(ahead / 16) * 64];
+        word1 = text[(size_t)(ahead / 16 + 1) * 64];
+    }
+};
+
+template <int BITS>
+using LetterSource = typename std::conditional<BITS == 2, Letters2, Cursor<BITS>>::type;
 
 
 // A plane of K 64-bit words shifted down one bit (bit i takes bit i + 1), each word's bit 63 taken from the next
@@ -231,9 +277,9 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     bool drained = false;
     int n = 0, m = 0, lo = 0, hi = 0, j = 0, top = 0;
     uint64_t pv[K], mv[K], pl[BITS][K];
-    Cursor<BITS> bottom, query;
-    bottom.park();
-    query.park();
+    LetterSource<BITS> bottom, query;
+    bottom.park(a.s.tape.words);
+    query.park(a.s.tape.words);
     uint8_t *trace = nullptr;
     bool recording = false;    // this group keeps the current pair's decisions
     long long trace_at = 0;    // and where, from the arena's start
@@ -300,8 +346,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
 #pragma unroll
                         for (int b = 0; b < BITS; ++b) pl[b][kk] = planes[b];
                     }
-                    bottom.begin(ref, -lo);
-                    query.begin(qry, 0);
+                    bottom.begin(ref, -lo, t & 7);
+                    query.begin(qry, 0, t & 7);
                     t0 = t;
                     if (TRACE) {
                         trace_at = a.trace_offset[slot] & ~127ll;
@@ -424,8 +470,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             }
             if (finishing) {
                 slot = -1;
-                bottom.park();
-                query.park();
+                bottom.park(a.s.tape.words);
+                query.park(a.s.tape.words);
             }
             continue;
         }
@@ -454,8 +500,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             uint32_t imv = L > 1 ? wave_neighbour<false>((uint32_t)mv[0]) : 0u;
             uint32_t ip0 = L > 1 ? wave_neighbour<false>((uint32_t)pl[0][0]) : 0u;
             uint32_t ip1 = L > 1 ? wave_neighbour<false>((uint32_t)pl[1][0]) : 0u;
+            const uint32_t code = bottom.next();  // every lane takes it; the band's bottom lane uses it
             if (last_lane) {
-                const uint32_t code = bottom.next();
                 ipv = 1u;
                 imv = 0u;
                 ip0 = code;
@@ -478,6 +524,10 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
         }
         // The rows whose letter equals the query's letter j - 1.
         const uint32_t c = query.next();
+        if (place == 7) {  // the period's last column: both sequences' next letters
+            bottom.refill();
+            query.refill();
+        }
         uint64_t eq[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
