@@ -283,6 +283,8 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     uint8_t *trace = nullptr;
     bool recording = false;    // this group keeps the current pair's decisions
     long long trace_at = 0;    // and where, from the arena's start
+    uint32_t line = 0;         // one-word lanes: the lane's first line, in 128-byte lines from the arena's start
+                               // (the arena is under 2^39 bytes: at most 0.6 of the device's memory)
     constexpr int WL = L * K;  // the pair's words
     // One word a lane: columns staged in LDS, eight places a lane, a place row padded by one lane so the wave's
     // line reads (eight lanes a line) fall in different banks.
@@ -368,6 +370,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                         recording = trace_at >= 0;
                     }
                     trace = recording ? a.arena + trace_at : a.arena;
+                    line = recording ? (uint32_t)(trace_at >> 7) + (uint32_t)k : 0u;
                     if (first_lane) {
                         atomicAdd(&a.stats[(TRACE ? EDIT_BINS : 0) + bin], 1u);
                         atomicAdd(&a.work[(TRACE ? EDIT_BINS : 0) + bin], (unsigned long long)m * (L * K));
@@ -590,18 +593,48 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     const unsigned long long writers = __ballot(active && recording);
                     if (writers) {
                         __builtin_amdgcn_wave_barrier();
-                        const uint64_t mine = (uint64_t)(uintptr_t)(trace + ((size_t)(((t - 1) >> 3) - (t0 >> 3)) * WL + k) * 128);
-                        const int piece = lane & 7;
+                        const uint32_t mine = line + (uint32_t)(((t - 1) >> 3) - (t0 >> 3)) * WL;
+                        const int piece = lane & 7, row = lane >> 3;
+                        const uint64_t base = (uint64_t)(uintptr_t)a.arena + (uint64_t)piece * 16;
+                        if constexpr (L >= 8) {
+                            // Lanes 8i .. 8i + 7 are one pair's, so lane 8i + g's line is lane 8i's plus g, and
+                            // the eight write or none do. With every pair writing, four lines' places are read
+                            // before their four stores, so the reads' waits overlap.
+                            if (writers == ~0ull) {
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            if (((writers >> (8 * i)) & 0xFFull) == 0) continue;
-                            const int from = 8 * i + (lane >> 3);
-                            const uint32_t at_lo = __shfl((uint32_t)mine, from);
-                            const uint32_t at_hi = __shfl((uint32_t)(mine >> 32), from);
-                            const ulonglong2 v = places[piece][from];
-                            if ((writers >> from) & 1ull)
-                                *reinterpret_cast<global_decisions *>((((uint64_t)at_hi << 32) | at_lo) +
-                                                                      (uint64_t)piece * 16) = planes_of(v);
+                                for (int half = 0; half < 2; ++half) {
+                                    ulonglong2 v[4];
+                                    uint32_t at[4];
+#pragma unroll
+                                    for (int i = 0; i < 4; ++i) {
+                                        const int g = 4 * half + i;
+                                        at[i] = __builtin_amdgcn_readlane(mine, 8 * g) + (uint32_t)row;
+                                        v[i] = places[piece][8 * g + row];
+                                    }
+#pragma unroll
+                                    for (int i = 0; i < 4; ++i)
+                                        *reinterpret_cast<global_decisions *>(base + (uint64_t)at[i] * 128) =
+                                            planes_of(v[i]);
+                                }
+                            } else {
+#pragma unroll
+                                for (int g = 0; g < 8; ++g) {
+                                    if (((writers >> (8 * g)) & 0xFFull) == 0) continue;
+                                    const uint32_t at = __builtin_amdgcn_readlane(mine, 8 * g) + (uint32_t)row;
+                                    *reinterpret_cast<global_decisions *>(base + (uint64_t)at * 128) =
+                                        planes_of(places[piece][8 * g + row]);
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int g = 0; g < 8; ++g) {
+                                if (((writers >> (8 * g)) & 0xFFull) == 0) continue;
+                                const int from = 8 * g + row;
+                                const uint32_t at = __shfl(mine, from);
+                                const ulonglong2 v = places[piece][from];
+                                if ((writers >> from) & 1ull)
+                                    *reinterpret_cast<global_decisions *>(base + (uint64_t)at * 128) = planes_of(v);
+                            }
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
