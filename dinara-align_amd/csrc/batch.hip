@@ -818,8 +818,8 @@ void side_by_side(DeviceContext *ctx, int mark_base, bool timed, int ev, bool ap
 
 // The run's launches: reset; every bin's first queue, the bins side by side; the retry queues, narrowest bin
 // first; then (alignments) every bin's tracing sweep side by side, and the walk. `timed` records an event pair
-// around each launch (not under capture).
-int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace) {
+// around each launch (not under capture); `may_wait` lets the host read queue counts back between launches.
+int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace, bool may_wait) {
     DeviceContext *ctx = b->ctx;
     const int slots = (int)b->slot_pair.size();
     launch_edit_reset(a, slots, ctx->stream);
@@ -851,16 +851,33 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace)
     EditArgs later = retry;
     later.lists = retry.file_lists;
     later.counts = retry.file_counts;
+    // Launched one by one (`may_wait`, not captured): the later retries begin at the narrowest bin whose queue holds
+    // a pair once the retries have ended (a later retry files only into wider bins), and a tracing sweep is made only
+    // for a queue that holds a pair; the counts are read back, the main stream waiting (the early walk goes on). An
+    // empty launch costs about 0.08 ms of the main stream. A launch not made has its timing events back to back.
+    int32_t pending[2 * EDIT_BINS] = {};
+    int from = 0;
+    if (may_wait) {
+        if (!b->edit_later_counts.read(ctx, pending, EDIT_BINS * 4)) return DINARA_OUT_OF_MEMORY;
+        while (from < EDIT_BINS && pending[from] == 0) ++from;
+    }
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
         const int grid = edit_grid(b, bin, b->retry_reach[bin]);
         if (!grid) continue;
         if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin), ctx->stream);
-        launch_edit_sweep(later, bin, false, grid, ctx->stream);
+        if (bin >= from) launch_edit_sweep(later, bin, false, grid, ctx->stream);
         if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin + 1), ctx->stream);
     }
     if (trace) {
-        side_by_side(ctx, EDIT_BINS + 1, timed, EV_TRACE, b->side_by_side_bins,
-                     [&](int bin) { return b->trace_reach[bin] > 0; },
+        if (may_wait && !b->trace_counts.read(ctx, pending, EDIT_BINS * 4)) return DINARA_OUT_OF_MEMORY;
+        const auto traced = [&](int bin) { return b->trace_reach[bin] > 0 && (!may_wait || pending[bin] > 0); };
+        if (timed)
+            for (int bin = 0; bin < EDIT_BINS; ++bin)
+                if (b->trace_reach[bin] > 0 && !traced(bin)) {
+                    (void)hipEventRecord(ctx->event(EV_TRACE + 2 * bin), ctx->stream);
+                    (void)hipEventRecord(ctx->event(EV_TRACE + 2 * bin + 1), ctx->stream);
+                }
+        side_by_side(ctx, EDIT_BINS + 1, timed, EV_TRACE, b->side_by_side_bins, traced,
                      [&](int bin, hipStream_t st) {
                          launch_edit_sweep(a, bin, true, edit_grid(b, bin, b->trace_reach[bin]), st);
                      });
@@ -911,7 +928,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
             hipGraph_t captured = nullptr;
             if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return DINARA_OUT_OF_MEMORY;
-            const int64_t code = enqueue_unit(b, a, false, trace);
+            const int64_t code = enqueue_unit(b, a, false, trace, false);
             const hipError_t ended = hipStreamEndCapture(ctx->stream, &captured);
             if (code || ended != hipSuccess) return DINARA_OUT_OF_MEMORY;
             const hipError_t made = hipGraphInstantiate(&b->edit_graph, captured, nullptr, nullptr, 0);
@@ -927,7 +944,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
         b->graph_used = 1;
     } else {
         if (hipEventRecord(ctx->start, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
-        if (const int64_t code = enqueue_unit(b, a, b->timed, trace)) return code;
+        if (const int64_t code = enqueue_unit(b, a, b->timed, trace, true)) return code;
     }
     if (hipEventRecord(ctx->sweep_stop, ctx->stream) != hipSuccess) return DINARA_OUT_OF_MEMORY;
     // Pairs whose decisions found no room: trace them in further passes, the arena emptied each time.
