@@ -756,6 +756,31 @@ EditArgs edit_args(dinara_batch *b) {
     return e;
 }
 
+// The slots walked a wave a pair: those whose longer side is at least EDIT_WALK_LONG letters, the first ones (slots
+// are longest first).
+int long_walks(const dinara_batch *b) {
+    int count = 0;
+    while (count < (int)b->rows.size() && std::max(b->rows[count], b->cols[count]) >= EDIT_WALK_LONG) ++count;
+    return count;
+}
+
+// The walk of every slot ready for it: the long slots on `aside` and the others on `stream` at once (forked and joined
+// by marks `mark` and `mark` + 1; `mark` + 2 marks both ended), then the CIGAR text on `stream`.
+void walk_slots(dinara_batch *b, const EditArgs &a, hipStream_t stream, hipStream_t aside, int mark, bool early) {
+    DeviceContext *ctx = b->ctx;
+    const int slots = (int)b->slot_pair.size(), longs = long_walks(b);
+    if (longs > 0) {
+        (void)hipEventRecord(ctx->mark(mark), stream);
+        (void)hipStreamWaitEvent(aside, ctx->mark(mark), 0);
+        launch_edit_walk_long(a, longs, aside, early);
+        (void)hipEventRecord(ctx->mark(mark + 1), aside);
+    }
+    launch_edit_walk_short(a, longs, slots, stream, early);
+    if (longs > 0) (void)hipStreamWaitEvent(stream, ctx->mark(mark + 1), 0);
+    (void)hipEventRecord(ctx->mark(mark + 2), stream);
+    launch_edit_text(a, slots, stream, early);
+}
+
 // Workgroups of a bin's launch that may meet `reach` pairs: enough for every one at once, at most `grid_per_cu` a
 // compute unit when the groups refill from the queue; 0 when it meets none.
 int edit_grid(const dinara_batch *b, int bin, int64_t reach) {
@@ -789,6 +814,9 @@ constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, 
               EV_LATER = 6 * EDIT_BINS + 2;
 // Marks (untimed events) of the early walk's fork and join; the bins side by side use marks 0 .. 4 * EDIT_BINS + 2.
 constexpr int EARLY_WALK = 5 * EDIT_BINS;
+// Marks of the walks' forks and joins (the long slots' launch beside the others'): the early walk's three, the last
+// walk's three.
+constexpr int WALK_MARKS = 5 * EDIT_BINS + 2;
 
 // Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
 // stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
@@ -834,7 +862,7 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
         (void)hipEventRecord(ctx->mark(EARLY_WALK), ctx->stream);
         hipStream_t walker = ctx->side(EDIT_BINS);
         (void)hipStreamWaitEvent(walker, ctx->mark(EARLY_WALK), 0);
-        launch_edit_walk(a, slots, walker, true);
+        walk_slots(b, a, walker, ctx->side(EDIT_BINS + 1), WALK_MARKS, true);
         (void)hipEventRecord(ctx->mark(EARLY_WALK + 1), walker);
     }
     // The first retries side by side, filing into the later set; the later retries one bin after another.
@@ -881,10 +909,13 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
                      [&](int bin, hipStream_t st) {
                          launch_edit_sweep(a, bin, true, edit_grid(b, bin, b->trace_reach[bin]), st);
                      });
-        if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(EARLY_WALK + 1), 0);
+        // The last walk follows the early walk's kernels (it writes moves of slots the early walk marked unwalked)
+        // and runs while the early walk's text is made; the run ends when both texts are made.
+        if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(WALK_MARKS + 2), 0);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
-        launch_edit_walk(a, slots, ctx->stream);
+        walk_slots(b, a, ctx->stream, ctx->side(EDIT_BINS + 1), WALK_MARKS + 3, false);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK + 1), ctx->stream);
+        if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(EARLY_WALK + 1), 0);
     }
     return hipPeekAtLastError() == hipSuccess ? 0 : DINARA_OUT_OF_MEMORY;
 }
@@ -960,7 +991,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
             for (int bin = 0; bin < EDIT_BINS; ++bin)
                 if (const int grid = edit_grid(b, bin, b->trace_reach[bin]))
                     launch_edit_sweep(a, bin, true, grid, ctx->stream);
-            launch_edit_walk(a, (int)slots, ctx->stream);
+            walk_slots(b, a, ctx->stream, ctx->side(EDIT_BINS + 1), WALK_MARKS + 3, false);
             int32_t left = 0;
             if (hipPeekAtLastError() != hipSuccess || !b->deferred_count.read(ctx, &left, 4)) return DINARA_OUT_OF_MEMORY;
             if (left >= deferred) return DINARA_OUT_OF_MEMORY;  // a pair larger than the whole arena

@@ -122,9 +122,16 @@ struct EditStart {
 // Host-side entry points of edit.hip. `grid` is the workgroups of each persistent launch (256 threads each).
 void launch_edit_reset(const EditArgs &a, int slots, hipStream_t stream);
 void launch_edit_sweep(const EditArgs &a, int bin, bool trace, int grid, hipStream_t stream);
+// The walk, in two launches that may run at once: the first `long_slots` slots (those whose longer side is at least
+// EDIT_WALK_LONG letters: slots are longest first) a wave a pair, and slots `first` .. `slots` - 1 a lane a pair.
 // `early`: walk the slots ready at the first pass's end (copied by launch_edit_snapshot), while the retries sweep on
-// other streams; it writes no flag those sweeps write. Otherwise walk every ready slot not walked early.
-void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream, bool early = false);
+// other streams; it writes no flag those sweeps write. Otherwise walk every ready slot not walked early, leaving the
+// early slots' moves as they are. Then, once both have ended, the CIGAR text of the slots they walked (the early
+// walk's text may be made while the last walk runs).
+void launch_edit_walk_long(const EditArgs &a, int long_slots, hipStream_t stream, bool early = false);
+void launch_edit_walk_short(const EditArgs &a, int first, int slots, hipStream_t stream, bool early = false);
+void launch_edit_text(const EditArgs &a, int slots, hipStream_t stream, bool early = false);
+constexpr int EDIT_WALK_LONG = 4096;
 void launch_edit_snapshot(const EditArgs &a, int slots, hipStream_t stream);
 // Every slot's first band and first queue from predictions of log(1 + cost) in device memory (learned seeds).
 void launch_edit_seed(const EditArgs &a, const float *prediction, double margin, double slope, const EditStart &start,

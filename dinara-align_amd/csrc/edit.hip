@@ -694,11 +694,11 @@ __device__ ulonglong2 walk_blank[8];  // zeros: global memory, like the arena (o
 // makes every diagonal step up to the next gap or the half span's edge in one go, makes the gap, and takes the
 // choices again for the new diagonal, until it leaves the span. The moves go to the slot's tape; the CIGAR text is
 // made from it afterwards (edit_text).
-__global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early) {
+__global__ __launch_bounds__(64) void edit_walk(EditArgs a, int first, int slots, int early) {
     __shared__ u32x4 span_stage[64][32];     // each lane's span, put together by the lanes that loaded it
     __shared__ ulonglong2 span_where[64][2];  // each lane's span: its four chunks' addresses
-    const int t = blockIdx.x * 64 + threadIdx.x;
-    if ((t & ~63) >= slots) return;  // whole waves only: the rounds are the wave's
+    const int t = first + blockIdx.x * 64 + threadIdx.x;
+    if (first + (int)blockIdx.x * 64 >= slots) return;  // whole waves only: the rounds are the wave's
     int slot = -1;
     if (t < slots) {
         if (early) {
@@ -870,17 +870,163 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int slots, int early
         for (; c > 0; --c) ops.put(3u);
         ops.close();
     }
-    if (t < slots) a.walk_moves[t] = slot >= 0 && !outside ? (int32_t)ops.count : -1;
+    // The last walk leaves the slots walked early as they are: their text may still be being made.
+    if (t < slots && (early || !a.walk_early[t])) a.walk_moves[t] = slot >= 0 && !outside ? (int32_t)ops.count : -1;
+}
+
+// The walk of a pair whose longer side is at least EDIT_WALK_LONG letters: one pair a wave, every lane on it. The
+// wave keeps 1024 columns (128 lines) of the word its path is in in LDS, and loads the 1024 before them meanwhile.
+// Along a diagonal, lane q takes the choice of the column q before the current one: two ballots give the next 64
+// columns' choices, the diagonal run is the count of trailing zeros of their gaps (choice 1 'D' or 2 'I', the two
+// planes unequal), and a gap turns the diagonal, for which the lanes read again. Every lane holds the same walk state
+// and writes the same moves. The moves and the rule are edit_walk's.
+constexpr int LONG_LINES = 128;
+constexpr int LONG_COLUMNS = 8 * LONG_LINES;
+
+// 16 bits spread to the even bits of 32: code 1 for each set bit, two bits a move.
+__device__ __forceinline__ uint32_t spread_bits(uint32_t x) {
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+
+__global__ __launch_bounds__(64) void edit_walk_long(EditArgs a, int count, int early) {
+    __shared__ u32x4 block[LONG_COLUMNS];  // the resident block: a place's two planes, 16 bytes, from its first line's
+    const int t = blockIdx.x, lane = threadIdx.x;
+    if (t >= count) return;
+    int slot = -1;
+    if (early) {
+        slot = a.walk_early[t] ? t : -1;  // read only: the retries' sweeps write `walk_ready` meanwhile
+    } else if (a.walk_early[t]) {
+        if (lane == 0) a.walk_ready[t] = 0;  // walked early
+    } else if (a.walk_ready[t]) {
+        slot = t;
+        if (lane == 0) a.walk_ready[t] = 0;
+    }
+    if (slot < 0) {
+        // The last walk leaves the slots walked early as they are: their text may still be being made.
+        if (lane == 0 && (early || !a.walk_early[t])) a.walk_moves[t] = -1;
+        return;
+    }
+    int r = a.s.tape.rows[slot], c = a.s.tape.cols[slot];
+    const int bin = a.s.walk_class[slot];
+    const int width = edit_width(bin), words = edit_lanes(bin) * edit_words(bin);
+    int i = a.s.walk_low[slot] + width - 1 - c + r;  // bit of the band's window: row r's diagonal c - r
+    const uint8_t *base = a.arena;
+    int phase = 0;  // the place of the pair's column 0 in its first line
+    if (r > 0 && c > 0) {
+        const int64_t at = a.trace_offset[slot];
+        base += at & ~int64_t(127);
+        phase = (int)(at & 7);
+    }
+    OpTape ops = op_tape(a.s, slot);
+    const bool eqx = a.s.eqx != 0;  // without eqx two unequal letters are 'M' like two equal ones
+    bool live = r > 0 && c > 0, outside = false;
+    if (live && (i < 0 || i >= width)) {  // a proven band holds every step; this pair's did not
+        if (lane == 0) a.s.status[slot] = 3;
+        outside = true;
+        live = false;
+    }
+    const int lines = (c + phase + 7) >> 3;  // the pair's lines of each word
+    int held_word = -1, held_low = 0;        // the resident block: its word and first line
+    int next_word = -1, next_low = 0;        // the block in flight
+    u32x4 pieces[LONG_LINES / 8];
+    const auto issue = [&](int word, int low) __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < LONG_LINES / 8; ++j) {
+            int line = low + 8 * j + (lane >> 3);
+            if (line >= lines) line = low;  // past the pair: any line of it
+            pieces[j] = *reinterpret_cast<global_chunk *>(
+                (uint64_t)(uintptr_t)(base + ((size_t)line * words + word) * 128) + (uint64_t)(lane & 7) * 16);
+        }
+        next_word = word;
+        next_low = low;
+    };
+    const auto hand_over = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < LONG_LINES / 8; ++j) block[64 * j + lane] = pieces[j];
+        __builtin_amdgcn_wave_barrier();
+        held_word = next_word;
+        held_low = next_low;
+        next_word = -1;
+    };
+    while (live) {
+        const int word = i >> 6;
+        const int line = (c - 1 + phase) >> 3;  // the current column's
+        if (held_word != word || line < held_low || line >= held_low + LONG_LINES) {
+            const int low = line - (LONG_LINES - 1) > 0 ? line - (LONG_LINES - 1) : 0;
+            if (next_word != word || next_low != low) issue(word, low);
+            hand_over();
+            if (low > 0) issue(word, low - LONG_LINES > 0 ? low - LONG_LINES : 0);
+        }
+        // Diagonal runs and gaps in the resident block, until the path leaves it or its word, or ends.
+        for (;;) {
+            const int place = c - 1 + phase - lane;  // lane q: the column q before the current one
+            const bool valid = lane < c && place >= held_low * 8;
+            uint32_t xb = 0, yb = 0;
+            if (valid) {
+                const uint32_t *column = reinterpret_cast<const uint32_t *>(&block[place - held_low * 8]);
+                const int h = (i & 63) >> 5;
+                xb = (column[h] >> (i & 31)) & 1u;
+                yb = (column[2 + h] >> (i & 31)) & 1u;
+            }
+            const unsigned long long xs = __ballot(xb != 0), ys = __ballot(yb != 0), vs = __ballot(valid);
+            const unsigned long long stops = (xs ^ ys) | ~vs;  // a gap, or the block's or the pair's edge
+            const int stop = stops ? __builtin_ctzll(stops) : 64;
+            int steps = stop < r ? stop : r;
+            steps = steps < c ? steps : c;
+            const unsigned long long unequal = eqx ? ~xs & ~ys : 0ull;
+            for (int done = 0; done < steps; done += 16) {
+                const int moves = steps - done < 16 ? steps - done : 16;
+                const uint32_t bits = (uint32_t)(unequal >> done) & (moves == 16 ? 0xFFFFu : (1u << moves) - 1u);
+                ops.put_many((uint64_t)spread_bits(bits), moves);
+            }
+            r -= steps;
+            c -= steps;
+            if (r == 0 || c == 0) {
+                live = false;
+                break;
+            }
+            if (stop == 64) continue;          // 64 diagonal moves: on along the diagonal
+            if (!((vs >> stop) & 1ull)) break;  // the block's edge: the block before it
+            const uint32_t choice = (uint32_t)((xs >> stop) & 1ull) | ((uint32_t)((ys >> stop) & 1ull) << 1);
+            ops.put(choice + 1u);  // 'D' (code 2) or 'I' (code 3)
+            const int deletion = choice == 1u, insertion = choice != 1u;
+            r -= deletion;
+            c -= insertion;
+            i += insertion - deletion;
+            if (r == 0 || c == 0) {
+                live = false;
+                break;
+            }
+            if (i < 0 || i >= width) {  // a proven band holds every step; this pair's did not
+                if (lane == 0) a.s.status[slot] = 3;
+                outside = true;
+                live = false;
+                break;
+            }
+            if ((i >> 6) != word || ((c - 1 + phase) >> 3) < held_low) break;
+        }
+    }
+    if (next_word >= 0) hand_over();  // the last loads: every load the wave made is waited for before it ends
+    if (!outside) {
+        for (; r > 0; --r) ops.put(2u);
+        for (; c > 0; --c) ops.put(3u);
+        ops.close();
+    }
+    if (lane == 0) a.walk_moves[t] = outside ? -1 : (int32_t)ops.count;
 }
 
 // The CIGAR text of the pairs the walk just made moves for: a wave a pair, all 64 lanes on its tape, four pairs a
 // workgroup, their room in the output claimed with one atomic.
-__global__ __launch_bounds__(256) void edit_text(EditArgs a, int slots) {
+// `early`: the slots the early walk took; otherwise the others (the two may run at once).
+__global__ __launch_bounds__(256) void edit_text(EditArgs a, int slots, int early) {
     __shared__ unsigned long long room[4];
     __shared__ unsigned long long start;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = blockIdx.x * 4 + wave;
-    const int moves = slot < slots ? a.walk_moves[slot] : -1;
+    const int moves = slot < slots && (a.walk_early[slot] != 0) == (early != 0) ? a.walk_moves[slot] : -1;
     const uint64_t *words = moves >= 0 ? op_tape(a.s, slot).words : nullptr;
     const int64_t length = moves >= 0 ? wave_cigar_text(a.s.eqx, a.s.forward_cigar, words, moves, nullptr) : 0;
     if (lane == 0) room[wave] = (unsigned long long)length;
@@ -1071,10 +1217,20 @@ void launch_edit_seed(const EditArgs &a, const float *prediction, double margin,
                            start, slots);
 }
 
-void launch_edit_walk(const EditArgs &a, int slots, hipStream_t stream, bool early) {
+void launch_edit_walk_long(const EditArgs &a, int long_slots, hipStream_t stream, bool early) {
+    if (long_slots > 0)
+        hipLaunchKernelGGL(edit_walk_long, dim3(long_slots), dim3(64), 0, stream, a, long_slots, early ? 1 : 0);
+}
+
+void launch_edit_walk_short(const EditArgs &a, int first, int slots, hipStream_t stream, bool early) {
+    if (slots > first)
+        hipLaunchKernelGGL(edit_walk, dim3((slots - first + 63) / 64), dim3(64), 0, stream, a, first, slots,
+                           early ? 1 : 0);
+}
+
+void launch_edit_text(const EditArgs &a, int slots, hipStream_t stream, bool early) {
     if (slots <= 0) return;
-    hipLaunchKernelGGL(edit_walk, dim3((slots + 63) / 64), dim3(64), 0, stream, a, slots, early ? 1 : 0);
-    hipLaunchKernelGGL(edit_text, dim3((slots + 3) / 4), dim3(256), 0, stream, a, slots);
+    hipLaunchKernelGGL(edit_text, dim3((slots + 3) / 4), dim3(256), 0, stream, a, slots, early ? 1 : 0);
 }
 
 void launch_edit_requeue(const EditArgs &a, const int32_t *held, int count, hipStream_t stream) {
