@@ -652,8 +652,12 @@ def run_local(args) -> int:
     return 0
 
 
-EDIT_BINS = 9
-EDIT_WIDTHS = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+def edit_bins(handle, b):
+    """The unit path's bins (the library's count) and their widths in diagonals, read from a batch."""
+    count = int(handle.dinara_batch_stat(b, 12))
+    return count, [int(handle.dinara_batch_stat(b, 1200 + i)) for i in range(count)]
+
+
 BIT_OPS_FIRST = 34   # int32 ops a 64-cell word of the bit-parallel step (17 on 64-bit words)
 BIT_OPS_TRACE = 56   # the same with the 2-bit decisions of the tracing sweep
 
@@ -805,9 +809,11 @@ def bin_profile(handle, b):
     handle.dinara_batch_set_graph(b, 0)
     if handle.dinara_batch_run(b) != 0:
         raise RuntimeError("profiling run failed")
-    prof = {k: [handle.dinara_batch_stat(b, base + i) for i in range(EDIT_BINS)]
+    bins, widths = edit_bins(handle, b)
+    prof = {k: [handle.dinara_batch_stat(b, base + i) for i in range(bins)]
             for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900),
                             ("sweep_word_columns", 1000), ("trace_word_columns", 1100))}
+    prof["widths"] = widths
     prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
     prof["run_ms"] = handle.dinara_batch_stat(b, 0)
     handle.dinara_batch_set_graph(b, 1)
@@ -1306,9 +1312,11 @@ def run_learned_profile(args) -> int:
             raise RuntimeError("learned run failed")
         print(f"[profile] learned run {step + 1}: {handle.dinara_batch_stat(b, 0):.2f} ms", file=sys.stderr,
               flush=True)
-    prof = {k: [round(handle.dinara_batch_stat(b, base + i), 4) for i in range(EDIT_BINS)]
+    bins, widths = edit_bins(handle, b)
+    prof = {k: [round(handle.dinara_batch_stat(b, base + i), 4) for i in range(bins)]
             for k, base in (("sweep_ms", 600), ("sweeps", 700), ("trace_ms", 800), ("traces", 900),
                             ("sweep_word_columns", 1000), ("trace_word_columns", 1100))}
+    prof["widths"] = widths
     prof["walk_ms"] = handle.dinara_batch_stat(b, 5)
     prof["run_ms"] = handle.dinara_batch_stat(b, 0)
     prof["first_band_proved"] = handle.dinara_batch_stat(b, 4) / pairs

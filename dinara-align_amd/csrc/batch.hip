@@ -647,7 +647,8 @@ int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t coun
 // ------------------------------------------------------------------ the unit-cost path
 
 // The unit path's first queues: each pair in the bin of its first band, no narrower than its length class's start
-// bin, placed around the start and end diagonals. Caller holds the device lock.
+// bin (under band doubling, the first such bin whose width is a power of two), placed around the start and end
+// diagonals. Caller holds the device lock.
 int64_t unit_lists(dinara_batch *b) {
     const int64_t slots = (int64_t)b->slot_pair.size();
     const size_t cap = (size_t)std::max<int64_t>(slots, 1);
@@ -658,7 +659,8 @@ int64_t unit_lists(dinara_batch *b) {
     for (int64_t s = 0; s < slots; ++s) {
         const int n = b->rows[s], m = b->cols[s];
         const int lo = b->first_low[s], hi = b->first_high[s];
-        const int bin = edit_bin_for(hi - lo + 1, b->start_bin[edit_length_class(std::max(n, m))]);
+        const int from = b->start_bin[edit_length_class(std::max(n, m))];
+        const int bin = b->doubling ? edit_power_bin_for(hi - lo + 1, from) : edit_bin_for(hi - lo + 1, from);
         if (bin < 0) continue;  // wider than the widest band: left unsettled
         b->edit_first_low[s] = edit_place(lo, hi, edit_width(bin), n, m);
         b->edit_first_bin[s] = bin;
@@ -1885,9 +1887,15 @@ int64_t dinara_batch_seed_learned_by_length(dinara_batch *b, dinara_mlp *net, do
 
 int64_t dinara_batch_set_retry(dinara_batch *b, int64_t doubling) {
     if (!b) return DINARA_INVALID_MODE;
+    const bool changed = b->doubling != (doubling != 0);
     b->doubling = doubling != 0;
     drop_graph(b);
-    return 0;
+    // The host-filed first queues follow the policy (doubling keeps to the powers of two); seeded ones are filed
+    // again by the next seeding.
+    if (!changed || !b->unit || b->slot_pair.empty()) return 0;
+    std::lock_guard<std::mutex> serial(b->ctx->lock);
+    DeviceGuard guard(b->ctx);
+    return unit_lists(b);
 }
 
 int64_t dinara_batch_set_graph(dinara_batch *b, int64_t on) {
@@ -1986,11 +1994,13 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 9: return b->walked_first;
         case 10: return b->full_pairs;
         case 11: return b->full_ms;
+        case 12: return EDIT_BINS;
         default: break;
     }
-    if (what >= 600 && what < 1200 && what % 100 < EDIT_BINS) {
+    if (what >= 600 && what < 1300 && what % 100 < EDIT_BINS) {
         const int bin = (int)(what % 100);
         switch (what / 100) {
+            case 12: return edit_width(bin);
             case 10: return b->sweep_work[bin];
             case 11: return b->trace_work[bin];
             case 6: return b->edit_ms[bin];

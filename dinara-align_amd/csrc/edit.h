@@ -22,17 +22,41 @@
 
 namespace dinara {
 
-// Bins: 1, 2, 4, ..., 64 lanes of one 64-bit word each (64 to 4096 diagonals), then 64 lanes of 2 and 4 words. Every
-// step is a doubling: a lane keeps one word wherever it can, since the words of a lane are swept one after another
-// and a long pair's column-after-column chain is what the run waits on.
-constexpr int EDIT_BINS = 9;
-__host__ __device__ constexpr int edit_lanes(int bin) { return bin < 7 ? 1 << bin : 64; }
-__host__ __device__ constexpr int edit_words(int bin) { return bin < 7 ? 1 : (bin == 7 ? 2 : 4); }
+// Bins: 1, 2, 3, 4, 6, 8, 12, 16, 21, 32 and 64 lanes of one 64-bit word each (64 to 4096 diagonals), then 64 lanes
+// of 2 and 4 words. A lane keeps one word wherever it can, since the words of a lane are swept one after another and
+// a long pair's column-after-column chain is what the run waits on. Between the powers of two, the lane counts fill
+// most of a wave (3 lanes: 21 pairs a wave, 6: 10, 12: 5, 21: 3), so a band is seldom much wider than its pair asks:
+// a wave's cost is its column steps, whatever its lanes hold.
+constexpr int EDIT_BINS = 13;
+__host__ __device__ constexpr int edit_lanes(int bin) {
+    switch (bin) {
+        case 0: return 1;
+        case 1: return 2;
+        case 2: return 3;
+        case 3: return 4;
+        case 4: return 6;
+        case 5: return 8;
+        case 6: return 12;
+        case 7: return 16;
+        case 8: return 21;
+        case 9: return 32;
+        default: return 64;
+    }
+}
+__host__ __device__ constexpr int edit_words(int bin) { return bin < 11 ? 1 : (bin == 11 ? 2 : 4); }
 __host__ __device__ constexpr int edit_width(int bin) { return 64 * edit_lanes(bin) * edit_words(bin); }
-// The bin twice as wide as `bin` (plain band doubling's next band), -1 past the widest.
+// Plain band doubling (the policy the learned band is measured against) keeps to the bins whose widths are powers of
+// two: the first at least `need` wide from `from` on, and then the one twice as wide as `bin` (from a seeded band of
+// another width, the first at least twice as wide; -1 past the widest).
+__host__ __device__ constexpr bool edit_power_bin(int bin) { return (edit_width(bin) & (edit_width(bin) - 1)) == 0; }
+__host__ __device__ inline int edit_power_bin_for(int need, int from) {
+    for (int bin = from; bin < EDIT_BINS; ++bin)
+        if (need <= edit_width(bin) && edit_power_bin(bin)) return bin;
+    return -1;
+}
 __host__ __device__ inline int edit_doubled(int bin) {
     for (int next = bin + 1; next < EDIT_BINS; ++next)
-        if (edit_width(next) == 2 * edit_width(bin)) return next;
+        if (edit_width(next) >= 2 * edit_width(bin)) return next;  // from a power of two: exactly twice
     return -1;
 }
 

@@ -257,8 +257,10 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     constexpr bool RECORD = MODE != 0;         // keeps decisions
     constexpr int W = 64 * L * K;
     const int lane = threadIdx.x & 63;
-    const int k = lane & (L - 1);
-    const bool first_lane = k == 0, last_lane = k == L - 1;
+    // A wave holds 64 / L pair groups of L lanes; the lanes past the last whole group never take a pair.
+    const int k = lane % L;
+    const bool in_group = lane < (64 / L) * L;
+    const bool first_lane = k == 0 && in_group, last_lane = k == L - 1 && in_group;
     const int32_t *queue = (TRACE ? a.trace_lists : a.lists) + (size_t)bin * a.capacity;
     int32_t *counts = TRACE ? a.trace_counts : a.counts;
     const int count = counts[bin];
@@ -415,8 +417,12 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     part += __popcll(pv[kk] & mask) - __popcll(mv[kk] & mask);
                 }
             }
+            // The group's sum, at its first lane (the only one that uses it).
 #pragma unroll
-            for (int d = L / 2; d >= 1; d /= 2) part += __shfl_xor(part, d);
+            for (int d = 1; d < L; d <<= 1) {
+                const int other = __shfl_down(part, d);
+                part += k + d < L ? other : 0;
+            }
             const int found = hi + m + top + part;  // `top` is the first lane's
             if constexpr (STAGED) {
                 // A pair ending before its line's last place: that line from LDS, by its own lane.
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                         const uint32_t mine = line + (uint32_t)(((t - 1) >> 3) - (t0 >> 3)) * WL;
                         const int piece = lane & 7, row = lane >> 3;
                         const uint64_t base = (uint64_t)(uintptr_t)a.arena + (uint64_t)piece * 16;
-                        if constexpr (L >= 8) {
+                        if constexpr (L % 8 == 0) {
                             // Lanes 8i .. 8i + 7 are one pair's, so lane 8i + g's line is lane 8i's plus g, and
                             // the eight write or none do. With every pair writing, four lines' places are read
                             // before their four stores, so the reads' waits overlap.
@@ -1179,12 +1185,16 @@ void sweep_bin(const EditArgs &a, int bin, int grid, hipStream_t stream) {
     switch (bin) {
         case 0: sweep_bits<1, 1, MODE>(a, bin, grid, stream); break;
         case 1: sweep_bits<2, 1, MODE>(a, bin, grid, stream); break;
-        case 2: sweep_bits<4, 1, MODE>(a, bin, grid, stream); break;
-        case 3: sweep_bits<8, 1, MODE>(a, bin, grid, stream); break;
-        case 4: sweep_bits<16, 1, MODE>(a, bin, grid, stream); break;
-        case 5: sweep_bits<32, 1, MODE>(a, bin, grid, stream); break;
-        case 6: sweep_bits<64, 1, MODE>(a, bin, grid, stream); break;
-        case 7: sweep_bits<64, 2, MODE>(a, bin, grid, stream); break;
+        case 2: sweep_bits<3, 1, MODE>(a, bin, grid, stream); break;
+        case 3: sweep_bits<4, 1, MODE>(a, bin, grid, stream); break;
+        case 4: sweep_bits<6, 1, MODE>(a, bin, grid, stream); break;
+        case 5: sweep_bits<8, 1, MODE>(a, bin, grid, stream); break;
+        case 6: sweep_bits<12, 1, MODE>(a, bin, grid, stream); break;
+        case 7: sweep_bits<16, 1, MODE>(a, bin, grid, stream); break;
+        case 8: sweep_bits<21, 1, MODE>(a, bin, grid, stream); break;
+        case 9: sweep_bits<32, 1, MODE>(a, bin, grid, stream); break;
+        case 10: sweep_bits<64, 1, MODE>(a, bin, grid, stream); break;
+        case 11: sweep_bits<64, 2, MODE>(a, bin, grid, stream); break;
         default: sweep_bits<64, 4, MODE>(a, bin, grid, stream); break;
     }
 }

@@ -227,9 +227,11 @@ void dinara_batch_free(dinara_batch *batch);
  * pair under a user band), 10-21 one pair a block of waves (512, 1024, 1536, 2048, 3072, 4096..8192 in steps of
  * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
- * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep); for bin b (64, 128, ..., 4096 diagonals, then 8192 and 16384) 600+b its sweep's time (ms, runs without
- * the graph), 700+b the sweeps it began, 800+b its tracing sweep's time, 900+b the traces it began, 1000+b and
- * 1100+b the word-columns (a pair's columns times its band's 64-bit words) its sweeps and traces covered. Every
+ * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep), 12 the number of
+ * bins; for bin b (widths 64, 128, 192, 256, 384, 512, 768, 1024, 1344, 2048, 4096, 8192, 16384 diagonals) 1200+b
+ * its width, 600+b its sweep's time (ms, runs without the graph), 700+b the sweeps it began, 800+b its tracing
+ * sweep's time, 900+b the traces it began, 1000+b and 1100+b the word-columns (a pair's columns times its band's
+ * 64-bit words) its sweeps and traces covered. Every
  * batch: 10 the pairs no band settled that the whole-matrix path swept after the run, 11 that path's time (ms,
  * host clock, included in no other figure). For roofline reporting. No counterpart in the reference. */
 double dinara_batch_stat(dinara_batch *batch, int64_t what);
@@ -267,8 +269,9 @@ int64_t dinara_batch_set_retry(dinara_batch *batch, int64_t doubling);
 /*
  * New: the unit-cost path. A batch under unit costs (mismatch = extension, no opening, deletions priced the same,
  * or all of it times one scale) with no user band is served by a banded bit-parallel sweep: pairs binned by band
- * width (64 x 2^b diagonals), each bin one persistent launch in which groups of lanes take pairs from a queue as
- * they finish, a pair the band cannot prove filed into a wider bin; alignments sweep each settled pair's band again
+ * width (64 diagonals a lane; lanes a pair 1 to 64, some between the powers of two, then 64 lanes of two and four
+ * words), each bin one persistent launch in which groups of lanes take pairs from a queue as they finish, a pair
+ * the band cannot prove filed into a wider bin; alignments sweep each settled pair's band again
  * keeping 2 bits a cell, then walk. No counterpart in the reference (its device edit distance is
  * device_edit.mojo:29-207, whole matrices, distances only).
  */
@@ -291,7 +294,8 @@ int64_t dinara_batch_set_persistent(dinara_batch *batch, int64_t on);
  * matrix without a band is what its CPU path fills, lanes.mojo:795-947). */
 int64_t dinara_batch_set_whole(dinara_batch *batch, int64_t on);
 /* The launch shape: persistent workgroups a compute unit (1..64), and for each class of the pairs' longer side
- * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..8); null keeps them. */
+ * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..12, in the widths' order
+ * given for dinara_batch_stat); null keeps them. */
 int64_t dinara_batch_set_tuning(dinara_batch *batch, int64_t grid_per_cu, const int64_t *start_bins);
 /* Each bin's first queue, and each bin's tracing sweep, on a stream of its own so the bins run side by side (1, the
  * default), or the bins one after another (0). */

@@ -21,8 +21,6 @@ from oracle import oracle as o
 pytestmark = pytest.mark.gpu
 
 EDIT = o.Costs.affine(1, 0, 1)
-# Band bins: 64, 128, 256, ..., 4096, 8192, 16384 diagonals.
-BINS = 9
 
 
 def mutated(text, rate, rng, alphabet="ACGT"):
@@ -69,6 +67,11 @@ class Batch:
 
     def stat(self, what):
         return self.h.dinara_batch_stat(self.ptr, what)
+
+    def widths(self):
+        """The unit path's bins, their widths in diagonals: 64 a lane, 1 to 64 lanes of one word, then 64 lanes of
+        two and four words."""
+        return [int(self.stat(1200 + k)) for k in range(int(self.stat(12)))]
 
     def run(self):
         assert self.h.dinara_batch_run(self.ptr) == 0
@@ -121,11 +124,40 @@ def test_bins_of_several_lanes_against_the_oracle(right):
     b = Batch(refs, qrys, 1, ties_right=right)
     try:
         got = b.run()
-        swept = [b.stat(700 + k) for k in range(BINS)]
+        widths = b.widths()
+        swept = [b.stat(700 + k) for k in range(len(widths))]
     finally:
         b.free()
     assert got == oracle_alignments(refs, qrys, right=right)
-    assert sum(1 for k in range(2, 7) if swept[k] > 0) >= 3, swept
+    assert sum(1 for w, n in zip(widths, swept) if 128 < w <= 4096 and n > 0) >= 3, swept
+
+
+@pytest.mark.parametrize("right,keep", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_bins_between_the_powers_of_two(right, keep):
+    # Bands of 192, 384, 768 and 1344 diagonals: 3, 6, 12 and 21 lanes a pair, a wave holding 21, 10, 5 and 3 pairs
+    # and leaving its last lanes idle. Seeded guesses place pairs of 1,000-1,800 letters in each; under keep_first the
+    # sweep that proves a band keeps its decisions, otherwise a second sweep traces it.
+    rng = random.Random(71 + 2 * right + keep)
+    refs, qrys = pairs_of(48, rng, (1000, 1800), (0.02, 0.05))
+    guesses = np.array([150, 300, 600, 1100] * 12, dtype=np.int64)
+    h = _lib.lib()
+    b = Batch(refs, qrys, 1, ties_right=right)
+    try:
+        h.dinara_batch_set_keep_first(b.ptr, keep)
+        assert h.dinara_batch_seed(b.ptr, guesses.ctypes.data) == 0
+        got = b.run()
+        widths = b.widths()
+        swept = {w: b.stat(700 + k) for k, w in enumerate(widths)}
+    finally:
+        b.free()
+    assert got == oracle_alignments(refs, qrys, right=right)
+    assert all(swept[w] > 0 for w in (192, 384, 768, 1344)), swept
+    b = Batch(refs, qrys, 0)
+    try:
+        assert h.dinara_batch_seed(b.ptr, guesses.ctypes.data) == 0
+        assert b.run() == [cost for cost, _ in got]
+    finally:
+        b.free()
 
 
 def test_words_of_two_and_four_a_lane():
@@ -137,22 +169,24 @@ def test_words_of_two_and_four_a_lane():
     b = Batch(refs, qrys, 0)
     try:
         got = b.run()
-        swept = [b.stat(700 + k) for k in range(BINS)]
+        widths = b.widths()
+        swept = [b.stat(700 + k) for k in range(len(widths))]
     finally:
         b.free()
     assert got == [o.edit_distance(r, q) for r, q in zip(refs, qrys)]
-    assert swept[7] + swept[8] > 0, swept
+    assert sum(n for w, n in zip(widths, swept) if w > 4096) > 0, swept
     # CIGARs through bands of two words a lane: a length difference of 5000 needs more than 4096 diagonals.
     r = refs[0][:6000]
     qs = [mutated(r[1000:2000], 0.1, rng), "".join(rng.choice("ACGT") for _ in range(900))]
     b = Batch([r, r], qs, 1)
     try:
         got = b.run()
-        traced = [b.stat(900 + k) for k in range(BINS)]
+        widths = b.widths()
+        traced = [b.stat(900 + k) for k in range(len(widths))]
     finally:
         b.free()
     assert got == oracle_alignments([r, r], qs)
-    assert traced[7] + traced[8] == 2, traced
+    assert sum(n for w, n in zip(widths, traced) if w > 4096) == 2, traced
 
 
 def test_byte_letters_and_scaled_costs():
@@ -289,7 +323,8 @@ def test_autotune_keeps_every_result():
     try:
         chosen = (ctypes.c_int64 * 7)()
         assert h.dinara_batch_autotune(b.ptr, chosen) == 0
-        assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < BINS for i in range(1, 6))
+        bins = len(b.widths())
+        assert chosen[0] in (2, 4, 8, 16) and all(0 <= chosen[i] < bins for i in range(1, 6))
         got = b.run()
     finally:
         b.free()
