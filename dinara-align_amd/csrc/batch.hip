@@ -767,18 +767,23 @@ int long_walks(const dinara_batch *b) {
 }
 
 // The walk of every slot ready for it: the long slots on `aside` and the others on `stream` at once (forked and joined
-// by marks `mark` and `mark` + 1; `mark` + 2 marks both ended), then the CIGAR text on `stream`.
+// by marks `mark` and `mark` + 1; `mark` + 2 marks both ended; one after the other when `aside` is `stream`), then
+// the CIGAR text on `stream`. `aside` is the
+// narrowest bin's stream, which no retry uses (retries go to wider bins): a stream of its own would be one more than
+// the hardware queues bench.py asks for, and a stream sharing a queue runs after the other's work.
 void walk_slots(dinara_batch *b, const EditArgs &a, hipStream_t stream, hipStream_t aside, int mark, bool early) {
     DeviceContext *ctx = b->ctx;
     const int slots = (int)b->slot_pair.size(), longs = long_walks(b);
-    if (longs > 0) {
+    if (longs > 0 && aside == stream) {
+        launch_edit_walk_long(a, longs, stream, early);
+    } else if (longs > 0) {
         (void)hipEventRecord(ctx->mark(mark), stream);
         (void)hipStreamWaitEvent(aside, ctx->mark(mark), 0);
         launch_edit_walk_long(a, longs, aside, early);
         (void)hipEventRecord(ctx->mark(mark + 1), aside);
     }
     launch_edit_walk_short(a, longs, slots, stream, early);
-    if (longs > 0) (void)hipStreamWaitEvent(stream, ctx->mark(mark + 1), 0);
+    if (longs > 0 && aside != stream) (void)hipStreamWaitEvent(stream, ctx->mark(mark + 1), 0);
     (void)hipEventRecord(ctx->mark(mark + 2), stream);
     launch_edit_text(a, slots, stream, early);
 }
@@ -864,7 +869,9 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
         (void)hipEventRecord(ctx->mark(EARLY_WALK), ctx->stream);
         hipStream_t walker = ctx->side(EDIT_BINS);
         (void)hipStreamWaitEvent(walker, ctx->mark(EARLY_WALK), 0);
-        walk_slots(b, a, walker, ctx->side(EDIT_BINS + 1), WALK_MARKS, true);
+        // The early walk's two launches one after the other: side by side they slow the retries beside them,
+        // whose long pairs are the run's tail.
+        walk_slots(b, a, walker, walker, WALK_MARKS, true);
         (void)hipEventRecord(ctx->mark(EARLY_WALK + 1), walker);
     }
     // The first retries side by side, filing into the later set; the later retries one bin after another.
@@ -915,7 +922,7 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
         // and runs while the early walk's text is made; the run ends when both texts are made.
         if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(WALK_MARKS + 2), 0);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
-        walk_slots(b, a, ctx->stream, ctx->side(EDIT_BINS + 1), WALK_MARKS + 3, false);
+        walk_slots(b, a, ctx->stream, ctx->side(0), WALK_MARKS + 3, false);
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK + 1), ctx->stream);
         if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(EARLY_WALK + 1), 0);
     }
@@ -993,7 +1000,7 @@ int64_t run_unit(dinara_batch *b, bool trace) {
             for (int bin = 0; bin < EDIT_BINS; ++bin)
                 if (const int grid = edit_grid(b, bin, b->trace_reach[bin]))
                     launch_edit_sweep(a, bin, true, grid, ctx->stream);
-            walk_slots(b, a, ctx->stream, ctx->side(EDIT_BINS + 1), WALK_MARKS + 3, false);
+            walk_slots(b, a, ctx->stream, ctx->side(0), WALK_MARKS + 3, false);
             int32_t left = 0;
             if (hipPeekAtLastError() != hipSuccess || !b->deferred_count.read(ctx, &left, 4)) return DINARA_OUT_OF_MEMORY;
             if (left >= deferred) return DINARA_OUT_OF_MEMORY;  // a pair larger than the whole arena
