@@ -946,11 +946,20 @@ def run_uneven(args) -> int:
             return lambda: handle.dinara_batch_seed_learned_by_length(b, net, ctypes.c_double(margin),
                                                                       ctypes.c_double(slope))
 
-        # On the tuning pairs: sweeps keep their decisions (a proven first band is walked at once), the launch shape
-        # tuned again for the seeds, then margin and slope, first-bin floors and graph or not, each timed.
+        # On the tuning pairs: sweeps keep their decisions (a proven first band is walked at once); graph or not
+        # first (the two forms schedule the bins differently, so the choices after it are timed in the chosen one),
+        # then the launch shape tuned again for the seeds, margin and slope, and first-bin floors, each timed.
         margin, slope = args.margin, args.slope
         handle.dinara_batch_set_tuning(tb, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, 0, 0))
         handle.dinara_batch_set_keep_first(tb, 1)
+        by_launch = {}
+        for graph in (1, 0):
+            handle.dinara_batch_set_graph(tb, graph)
+            by_launch[graph] = timed_steps(handle, tb, 3, 1, before=seeder(tb, margin, slope))[0]
+            print(f"[bench] held-out: learned {'graph' if graph else 'launches one by one'}: {by_launch[graph]:.1f} ms",
+                  file=sys.stderr, flush=True)
+        use_graph = min(by_launch, key=by_launch.get)
+        handle.dinara_batch_set_graph(tb, use_graph)
         seeder(tb, margin, slope)()
         learned_chosen = (ctypes.c_int64 * 7)()
         if handle.dinara_batch_autotune(tb, learned_chosen) != 0:
@@ -961,7 +970,7 @@ def run_uneven(args) -> int:
             margins = [float(v) for v in args.margins.split(",")] if args.margins else [args.margin]
             for mg in margins:
                 for sl in slopes:
-                    by_slope[(mg, sl)] = timed_steps(handle, tb, 2, 1, before=seeder(tb, mg, sl))[0]
+                    by_slope[(mg, sl)] = timed_steps(handle, tb, 3, 1, before=seeder(tb, mg, sl))[0]
                     print(f"[bench] held-out: margin {mg} slope {sl}: {by_slope[(mg, sl)]:.1f} ms", file=sys.stderr,
                           flush=True)
             margin, slope = min(by_slope, key=by_slope.get)
@@ -974,13 +983,6 @@ def run_uneven(args) -> int:
                 by_floor[(f3, f4)] = timed_steps(handle, tb, 2, 1, before=seeder(tb, margin, slope))[0]
             floor = min(by_floor, key=by_floor.get)
             handle.dinara_batch_set_tuning(tb, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
-        by_launch = {}
-        for graph in (1, 0):
-            handle.dinara_batch_set_graph(tb, graph)
-            by_launch[graph] = timed_steps(handle, tb, 3, 1, before=seeder(tb, margin, slope))[0]
-            print(f"[bench] held-out: learned {'graph' if graph else 'launches one by one'}: {by_launch[graph]:.1f} ms",
-                  file=sys.stderr, flush=True)
-        use_graph = min(by_launch, key=by_launch.get)
     handle.dinara_batch_free(tb)
     del tkeep, tune_r, tune_q
 

@@ -261,6 +261,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     const int k = lane % L;
     const bool in_group = lane < (64 / L) * L;
     const bool first_lane = k == 0 && in_group, last_lane = k == L - 1 && in_group;
+    const unsigned long long inner_lanes = ~__ballot(last_lane);  // lanes whose carry goes on to the next lane
     const int32_t *queue = (TRACE ? a.trace_lists : a.lists) + (size_t)bin * a.capacity;
     int32_t *counts = TRACE ? a.trace_counts : a.counts;
     const int count = counts[bin];
@@ -542,25 +543,37 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
         for (int kk = 0; kk < K; ++kk) {
             uint64_t differ = 0;
 #pragma unroll
-            for (int b = 0; b < BITS; ++b) differ |= pl[b][kk] ^ (((c >> b) & 1u) ? ~0ull : 0ull);
+            for (int b = 0; b < BITS; ++b) {
+                const uint32_t m = 0u - ((c >> b) & 1u);  // bit b of the letter, in every bit (both halves alike)
+                differ |= pl[b][kk] ^ (((uint64_t)m << 32) | m);
+            }
             eq[kk] = ~differ;
         }
         // (Eq & Pv) + Pv over the pair's whole window: carries ripple through a lane's words, and across lanes by
         // generate / propagate ballots (a carry into lane t is bit t of (G + (G | P)) ^ P).
         uint64_t sum[K];
-        uint32_t carry = 0;
-        bool all_ones = true;
+        unsigned long long gen = 0, prop = 0;  // lanes whose words carry out, lanes whose words pass a carry on
+        if constexpr (K == 1) {
+            // One word: the ballots of the compares themselves, the groups' last lanes masked off in the scalar unit.
+            sum[0] = (eq[0] & pv[0]) + pv[0];
+            // The compares' lane masks (llvm.amdgcn.icmp, predicates 36 unsigned less, 32 equal).
+            gen = __builtin_amdgcn_uicmpl(sum[0], pv[0], 36) & inner_lanes;
+            prop = __builtin_amdgcn_uicmpl(sum[0], ~0ull, 32) & inner_lanes;
+        } else {
+            uint32_t carry = 0;
+            bool all_ones = true;
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-            const uint64_t x = (eq[kk] & pv[kk]) + pv[kk];
-            const uint32_t c1 = x < pv[kk];
-            sum[kk] = x + carry;
-            carry = c1 | (uint32_t)(carry && x == ~0ull);
-            all_ones = all_ones && sum[kk] == ~0ull;
+            for (int kk = 0; kk < K; ++kk) {
+                const uint64_t x = (eq[kk] & pv[kk]) + pv[kk];
+                const uint32_t c1 = x < pv[kk];
+                sum[kk] = x + carry;
+                carry = c1 | (uint32_t)(carry && x == ~0ull);
+                all_ones = all_ones && sum[kk] == ~0ull;
+            }
+            gen = __builtin_amdgcn_ballot_w64(carry != 0) & inner_lanes;
+            prop = __builtin_amdgcn_ballot_w64(all_ones) & inner_lanes;
         }
         if constexpr (L > 1) {
-            const unsigned long long gen = __builtin_amdgcn_ballot_w64(carry != 0 && !last_lane);
-            const unsigned long long prop = __builtin_amdgcn_ballot_w64(all_ones && !last_lane);
             const unsigned long long into = (gen + (gen | prop)) ^ prop;
             if ((into >> lane) & 1ull) {
                 uint32_t up = 1;
@@ -664,7 +677,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             pv[kk] = npv;
             mv[kk] = nmv;
         }
-        top += (int)(pv[0] & 1) - (int)(mv[0] & 1);
+        top += (int)((uint32_t)pv[0] & 1u) + __builtin_amdgcn_sbfe((int)(uint32_t)mv[0], 0, 1);
         }
     }
 }
