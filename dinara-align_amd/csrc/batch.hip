@@ -710,7 +710,7 @@ int64_t unit_buffers(dinara_batch *b, const std::vector<int32_t> &lists, const s
         !b->edit_first_low_d.upload(ctx, b->edit_first_low.data(), cap * 4) || !b->edit_band_low.reserve(cap * 4) ||
         !b->edit_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->edit_retry_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_retry_counts.reserve(2 * EDIT_BINS * 4) ||
-        !b->edit_later_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_later_counts.reserve(2 * EDIT_BINS * 4) ||
+        !b->edit_later_lists.reserve(EDIT_BINS * cap * 4) || !b->edit_later_counts.reserve(3 * EDIT_BINS * 4) ||
         !b->trace_lists.reserve(EDIT_BINS * cap * 4) || !b->trace_counts.reserve(2 * EDIT_BINS * 4) ||
         !b->trace_offset.reserve(cap * 8) || !b->walk_ready.reserve(cap * 4) ||
         !b->walk_moves.reserve(cap * 4) || !b->walk_early.reserve(cap * 4) ||
@@ -892,10 +892,16 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
     // a pair once the retries have ended (a later retry files only into wider bins), and a tracing sweep is made only
     // for a queue that holds a pair; the counts are read back, the main stream waiting (the early walk goes on). An
     // empty launch costs about 0.08 ms of the main stream. A launch not made has its timing events back to back.
-    int32_t pending[2 * EDIT_BINS] = {};
+    // One read brings both sets of counts: the tracing queues' are copied beside the later retries' first (the
+    // later-retry counts' buffer has room past its lengths and heads), and when no later retry runs they are final.
+    int32_t pending[3 * EDIT_BINS] = {};
     int from = 0;
     if (may_wait) {
-        if (!b->edit_later_counts.read(ctx, pending, EDIT_BINS * 4)) return DINARA_OUT_OF_MEMORY;
+        int32_t *later_counts = static_cast<int32_t *>(b->edit_later_counts.ptr);
+        if (trace && hipMemcpyAsync(later_counts + 2 * EDIT_BINS, b->trace_counts.ptr, EDIT_BINS * 4,
+                                    hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            return DINARA_OUT_OF_MEMORY;
+        if (!b->edit_later_counts.read(ctx, pending, sizeof(pending))) return DINARA_OUT_OF_MEMORY;
         while (from < EDIT_BINS && pending[from] == 0) ++from;
     }
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
@@ -906,8 +912,10 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
         if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin + 1), ctx->stream);
     }
     if (trace) {
-        if (may_wait && !b->trace_counts.read(ctx, pending, EDIT_BINS * 4)) return DINARA_OUT_OF_MEMORY;
-        const auto traced = [&](int bin) { return b->trace_reach[bin] > 0 && (!may_wait || pending[bin] > 0); };
+        int32_t *const trace_pending = pending + 2 * EDIT_BINS;
+        if (may_wait && from < EDIT_BINS && !b->trace_counts.read(ctx, trace_pending, EDIT_BINS * 4))
+            return DINARA_OUT_OF_MEMORY;
+        const auto traced = [&](int bin) { return b->trace_reach[bin] > 0 && (!may_wait || trace_pending[bin] > 0); };
         if (timed)
             for (int bin = 0; bin < EDIT_BINS; ++bin)
                 if (b->trace_reach[bin] > 0 && !traced(bin)) {
