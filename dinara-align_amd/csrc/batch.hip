@@ -219,7 +219,11 @@ struct dinara_batch {
     ~dinara_batch() {
         for (DeviceBuffer *buffer : arenas) delete buffer;
         if (edit_graph) (void)hipGraphExecDestroy(edit_graph);
+        if (pinned_counts) (void)hipHostFree(pinned_counts);
     }
+    // Alignments launched one by one: the queue counts a run reads back while its last walk runs (pinned, made at
+    // the first such run).
+    int32_t *pinned_counts = nullptr;
     // the last run's results, by slot
     std::vector<int32_t> cost_h, status_h, cigar_length_h;
     std::vector<int64_t> cigar_offset_h;
@@ -822,8 +826,9 @@ constexpr int EV_FIRST = 0, EV_RETRY = 2 * EDIT_BINS, EV_TRACE = 4 * EDIT_BINS, 
 // Marks (untimed events) of the early walk's fork and join; the bins side by side use marks 0 .. 4 * EDIT_BINS + 2.
 constexpr int EARLY_WALK = 5 * EDIT_BINS;
 // Marks of the walks' forks and joins (the long slots' launch beside the others'): the early walk's three, the last
-// walk's three.
+// walk's three; then the queue counts' read-back.
 constexpr int WALK_MARKS = 5 * EDIT_BINS + 2;
+constexpr int COUNTS_MARK = WALK_MARKS + 6;
 
 // Runs `launch(bin, stream)` for each bin `use(bin)` picks, every one on a stream of its own forked from the main
 // stream and joined back to it (`mark_base` picks the ordering events), each between timing events EV + 2 bin and
@@ -892,16 +897,41 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
     // a pair once the retries have ended (a later retry files only into wider bins), and a tracing sweep is made only
     // for a queue that holds a pair; the counts are read back, the main stream waiting (the early walk goes on). An
     // empty launch costs about 0.08 ms of the main stream. A launch not made has its timing events back to back.
-    // One read brings both sets of counts: the tracing queues' are copied beside the later retries' first (the
-    // later-retry counts' buffer has room past its lengths and heads), and when no later retry runs they are final.
+    // Launched one by one (`may_wait`, not captured), the later retries begin at the narrowest bin whose queue holds
+    // a pair once the retries have ended (a later retry files only into wider bins), and a tracing sweep is made only
+    // for a queue that holds a pair: one read brings both sets of counts, the tracing queues' copied beside the
+    // later retries' first (the later-retry counts' buffer has room past its lengths and heads). An empty launch
+    // costs about 0.08 ms of the main stream. A launch not made has its timing events back to back.
+    //
+    // Alignments launched one by one walk the pairs ready once the retries end at once, while the counts come back
+    // (pinned, waited for on an event): the host's round trip, about 0.6 ms, is then hidden under that walk. Only
+    // when a later retry or a tracing sweep runs does a second walk follow them.
     int32_t pending[3 * EDIT_BINS] = {};
+    int32_t *const trace_pending = pending + 2 * EDIT_BINS;
     int from = 0;
+    bool walked = false;
     if (may_wait) {
         int32_t *later_counts = static_cast<int32_t *>(b->edit_later_counts.ptr);
         if (trace && hipMemcpyAsync(later_counts + 2 * EDIT_BINS, b->trace_counts.ptr, EDIT_BINS * 4,
                                     hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
             return DINARA_OUT_OF_MEMORY;
-        if (!b->edit_later_counts.read(ctx, pending, sizeof(pending))) return DINARA_OUT_OF_MEMORY;
+        if (trace && !b->pinned_counts &&
+            hipHostMalloc(reinterpret_cast<void **>(&b->pinned_counts), sizeof(pending)) != hipSuccess)
+            b->pinned_counts = nullptr;
+        if (trace && b->pinned_counts) {
+            if (hipMemcpyAsync(b->pinned_counts, later_counts, sizeof(pending), hipMemcpyDeviceToHost, ctx->stream) !=
+                    hipSuccess ||
+                hipEventRecord(ctx->mark(COUNTS_MARK), ctx->stream) != hipSuccess)
+                return DINARA_OUT_OF_MEMORY;
+            if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(WALK_MARKS + 2), 0);
+            if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
+            walk_slots(b, a, ctx->stream, ctx->side(0), WALK_MARKS + 3, false);
+            walked = true;
+            if (hipEventSynchronize(ctx->mark(COUNTS_MARK)) != hipSuccess) return DINARA_OUT_OF_MEMORY;
+            std::memcpy(pending, b->pinned_counts, sizeof(pending));
+        } else if (!b->edit_later_counts.read(ctx, pending, sizeof(pending))) {
+            return DINARA_OUT_OF_MEMORY;
+        }
         while (from < EDIT_BINS && pending[from] == 0) ++from;
     }
     for (int bin = 0; bin < EDIT_BINS; ++bin) {
@@ -912,10 +942,11 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
         if (timed) (void)hipEventRecord(ctx->event(EV_LATER + 2 * bin + 1), ctx->stream);
     }
     if (trace) {
-        int32_t *const trace_pending = pending + 2 * EDIT_BINS;
         if (may_wait && from < EDIT_BINS && !b->trace_counts.read(ctx, trace_pending, EDIT_BINS * 4))
             return DINARA_OUT_OF_MEMORY;
         const auto traced = [&](int bin) { return b->trace_reach[bin] > 0 && (!may_wait || trace_pending[bin] > 0); };
+        bool any_traced = false;
+        for (int bin = 0; bin < EDIT_BINS; ++bin) any_traced = any_traced || traced(bin);
         if (timed)
             for (int bin = 0; bin < EDIT_BINS; ++bin)
                 if (b->trace_reach[bin] > 0 && !traced(bin)) {
@@ -927,10 +958,14 @@ int64_t enqueue_unit(dinara_batch *b, const EditArgs &a, bool timed, bool trace,
                          launch_edit_sweep(a, bin, true, edit_grid(b, bin, b->trace_reach[bin]), st);
                      });
         // The last walk follows the early walk's kernels (it writes moves of slots the early walk marked unwalked)
-        // and runs while the early walk's text is made; the run ends when both texts are made.
-        if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(WALK_MARKS + 2), 0);
-        if (timed) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
-        walk_slots(b, a, ctx->stream, ctx->side(0), WALK_MARKS + 3, false);
+        // and runs while the early walk's text is made; the run ends when both texts are made. After a walk already
+        // made, a second one only for what later retries and tracing sweeps proved (it marks the first's slots
+        // unwalked, whose text the first made before it on this stream).
+        if (!walked || from < EDIT_BINS || any_traced) {
+            if (early && !walked) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(WALK_MARKS + 2), 0);
+            if (timed && !walked) (void)hipEventRecord(ctx->event(EV_WALK), ctx->stream);
+            walk_slots(b, a, ctx->stream, ctx->side(0), WALK_MARKS + 3, false);
+        }
         if (timed) (void)hipEventRecord(ctx->event(EV_WALK + 1), ctx->stream);
         if (early) (void)hipStreamWaitEvent(ctx->stream, ctx->mark(EARLY_WALK + 1), 0);
     }
