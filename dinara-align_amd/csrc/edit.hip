@@ -289,11 +289,12 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
     uint32_t line = 0;         // one-word lanes: the lane's first line, in 128-byte lines from the arena's start
                                // (the arena is under 2^39 bytes: at most 0.6 of the device's memory)
     constexpr int WL = L * K;  // the pair's words
-    // One word a lane: columns staged in LDS, eight places a lane, a place row padded by one lane so the wave's
-    // line reads (eight lanes a line) fall in different banks.
+    // One word a lane: columns staged in LDS, eight places a lane, lane l's place p at index l ^ p of place row p, so
+    // the wave's line reads (eight lanes a line, one place each) fall in different banks with no padding: 32 KB a
+    // workgroup, five to a compute unit.
     constexpr bool STAGED = RECORD && K == 1;
-    __shared__ ulonglong2 stage[STAGED ? 4 : 1][STAGED ? 8 : 1][STAGED ? 65 : 1];
-    ulonglong2 (*const places)[STAGED ? 65 : 1] = stage[STAGED ? threadIdx.x >> 6 : 0];
+    __shared__ ulonglong2 stage[STAGED ? 4 : 1][STAGED ? 8 : 1][STAGED ? 64 : 1];
+    ulonglong2 (*const places)[STAGED ? 64 : 1] = stage[STAGED ? threadIdx.x >> 6 : 0];
     int t = 0;       // the wave's column steps so far (the same in every lane)
     int t0 = 0;      // the step count when the lane's pair was claimed
     // More words a lane: the last HOLD columns' decision planes in registers, oldest first.
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     uint8_t *line = trace + ((size_t)(((t - 1) >> 3) - (t0 >> 3)) * WL + k) * 128;
                     for (int q = 0; q <= last; ++q)
                         reinterpret_cast<global_decisions *>((uint64_t)(uintptr_t)line)[q] =
-                            planes_of(places[q][lane]);
+                            planes_of(places[q][lane ^ q]);
                 }
             } else if constexpr (RECORD) {
                 // The last, partly filled columns: shifted down to the chunk's start, then written whole.
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             if constexpr (STAGED) {
                 // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
                 const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
-                places[place][lane] = make_ulonglong2(~unequal_step & (npv | ~ph[kk]), ~unequal_step & ~npv);
+                places[place][lane ^ place] = make_ulonglong2(~unequal_step & (npv | ~ph[kk]), ~unequal_step & ~npv);
                 if (place == 7) {
                     // Every lane's line is full: the wave writes them, lanes 8g .. 8g + 7 the eight places of lane
                     // g' = 8i + g's line in instruction i.
@@ -628,7 +629,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                                     for (int i = 0; i < 4; ++i) {
                                         const int g = 4 * half + i;
                                         at[i] = __builtin_amdgcn_readlane(mine, 8 * g) + (uint32_t)row;
-                                        v[i] = places[piece][8 * g + row];
+                                        v[i] = places[piece][(8 * g + row) ^ piece];
                                     }
 #pragma unroll
                                     for (int i = 0; i < 4; ++i)
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                                     if (((writers >> (8 * g)) & 0xFFull) == 0) continue;
                                     const uint32_t at = __builtin_amdgcn_readlane(mine, 8 * g) + (uint32_t)row;
                                     *reinterpret_cast<global_decisions *>(base + (uint64_t)at * 128) =
-                                        planes_of(places[piece][8 * g + row]);
+                                        planes_of(places[piece][(8 * g + row) ^ piece]);
                                 }
                             }
                         } else {
@@ -650,7 +651,7 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                                 if (((writers >> (8 * g)) & 0xFFull) == 0) continue;
                                 const int from = 8 * g + row;
                                 const uint32_t at = __shfl(mine, from);
-                                const ulonglong2 v = places[piece][from];
+                                const ulonglong2 v = places[piece][from ^ piece];
                                 if ((writers >> from) & 1ull)
                                     *reinterpret_cast<global_decisions *>(base + (uint64_t)at * 128) = planes_of(v);
                             }
