@@ -232,10 +232,10 @@ __device__ void refile(const EditArgs &a, int slot, int n, int m, int lo, int hi
 // decisions of each band as it goes (SWEEP_TRACED), so a pair its first band proves goes straight to the walk.
 constexpr int SWEEP = 0, TRACE = 1, SWEEP_TRACED = 2;
 
-// Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. Kept decisions are 2 bits a cell (0
-// two unequal letters, 1 a reference letter alone, 2 a query letter alone, 3 two equal letters: the walk's first
-// true choice in that order) in two bit-planes: word w's columns 8q..8q+7 in the 128 bytes at arena + offset +
-// (q words + w) 128, 16 a column.
+// Bin `bin`'s persistent sweep: L lanes of K words a pair, letters BITS a code. Kept decisions are 2 bits a cell, the
+// walk's first true choice in the order two unequal letters (3: both planes), a reference letter alone (1: plane 0),
+// a query letter alone (2: plane 1), two equal letters (0), in two bit-planes: word w's columns 8q..8q+7 in the 128
+// bytes at arena + offset + (q words + w) 128, 16 a column.
 //
 // A lane of one word (K = 1) writes whole lines in step with its wave: a pair claimed when its wave has made t0
 // column steps starts its columns at place `phase` = t0 mod 8 of its first line (its column j at place phase + j of
@@ -604,9 +604,10 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
             const uint64_t npv = mhs | ~(xv[kk] | phs);
             const uint64_t nmv = phs & xv[kk];
             if constexpr (STAGED) {
-                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
-                const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
-                places[place][lane ^ place] = make_ulonglong2(~unequal_step & (npv | ~ph[kk]), ~unequal_step & ~npv);
+                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1:
+                // one of them +1, neither -1). The planes: X or a reference letter alone; X or a query letter alone.
+                const uint64_t unequal_step = ~eq[kk] & (npv ^ phs) & ~(nmv | mhs);
+                places[place][lane ^ place] = make_ulonglong2(unequal_step | npv, unequal_step | (ph[kk] & ~npv));
                 if (place == 7) {
                     // Every lane's line is full: the wave writes them, lanes 8g .. 8g + 7 the eight places of lane
                     // g' = 8i + g's line in instruction i.
@@ -660,15 +661,16 @@ __global__ __launch_bounds__(256) void edit_sweep(EditArgs a, int bin) {
                     }
                 }
             } else if constexpr (RECORD) {
-                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1).
-                const uint64_t unequal_step = ~eq[kk] & ((npv & ~(phs | mhs)) | (phs & ~(npv | nmv)));
+                // X: unequal letters whose diagonal neighbour is one less (vertical + horizontal difference 1:
+                // one of them +1, neither -1).
+                const uint64_t unequal_step = ~eq[kk] & (npv ^ phs) & ~(nmv | mhs);
 #pragma unroll
                 for (int q = 0; q + 1 < HOLD; ++q) {
                     held0[q][kk] = held0[q + 1][kk];
                     held1[q][kk] = held1[q + 1][kk];
                 }
-                held0[HOLD - 1][kk] = ~unequal_step & (npv | ~ph[kk]);
-                held1[HOLD - 1][kk] = ~unequal_step & ~npv;
+                held0[HOLD - 1][kk] = unequal_step | npv;
+                held1[HOLD - 1][kk] = unequal_step | (ph[kk] & ~npv);
                 if (active && recording && (j % HOLD) == 0) {
                     global_decisions *chunk = reinterpret_cast<global_decisions *>((uint64_t)(uintptr_t)trace + held_chunk_at<HOLD>(j / HOLD - 1, WL, k * K + kk));
 #pragma unroll
@@ -845,7 +847,7 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int first, int slots
             const int span = (c - 1 + phase) >> 5, word = i >> 6;
             for (;;) {
                 // The diagonal run from column q down to the nearest gap (choice 1 'D' or 2 'I') or the half span's
-                // edge, then the gap: one batch of moves. Choice 0 (two unequal letters) is 'X' with eqx, 3 '='.
+                // edge, then the gap: one batch of moves. Choice 3 (two unequal letters) is 'X' with eqx, 0 '='.
                 const int q = (c - 1 + phase) & 15;
                 const uint32_t upto = q == 15 ? 0xFFFFFFFFu : (4u << (2 * q)) - 1u;
                 const uint32_t gaps = (choices ^ (choices >> 1)) & 0x55555555u & upto;
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(64) void edit_walk(EditArgs a, int first, int slots
                 // No further than the rows left, nor than the columns left (the places before the first are none).
                 const int run = q - stop < r ? q - stop : r;
                 const int steps = run < c ? run : c;
-                const uint32_t unequal = ~choices & ~(choices >> 1) & unequal_marks;
+                const uint32_t unequal = choices & (choices >> 1) & unequal_marks;
                 const uint32_t walked = (__builtin_bitreverse32(unequal) >> 1) >> (2 * (15 - q));
                 uint64_t codes = steps >= 16 ? (uint64_t)walked : (uint64_t)(walked & ((1u << (2 * steps)) - 1u));
                 r -= steps;
@@ -996,7 +998,7 @@ __global__ __launch_bounds__(64) void edit_walk_long(EditArgs a, int count, int 
             const int stop = stops ? __builtin_ctzll(stops) : 64;
             int steps = stop < r ? stop : r;
             steps = steps < c ? steps : c;
-            const unsigned long long unequal = eqx ? ~xs & ~ys : 0ull;
+            const unsigned long long unequal = eqx ? xs & ys : 0ull;
             for (int done = 0; done < steps; done += 16) {
                 const int moves = steps - done < 16 ? steps - done : 16;
                 const uint32_t bits = (uint32_t)(unequal >> done) & (moves == 16 ? 0xFFFFu : (1u << moves) - 1u);
