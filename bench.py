@@ -66,7 +66,7 @@ CLASSES = len(CLASS_NAMES)
 # workload (200,000 pairs), every edit_sweep dispatch of one run summed (the bins run side by side):
 # profiles/edit_sweep_learned_uneven_200k_hbm_counters.md.
 MEASURED_TRAFFIC = {("affine", 100_000, "sweep_group<2>"): {"read": 119.1e6, "written": 823.3e6},
-                    ("uneven", 200_000, "edit_sweep"): {"read": 2.558e9, "written": 98.25e9}}
+                    ("uneven", 200_000, "edit_sweep"): {"read": 2.439e9, "written": 87.46e9}}
 
 
 def host_cores() -> int:
