@@ -260,10 +260,38 @@ def test_launch_shapes_change_no_result():
         "keep first, doubling, one pair a group": lambda b: (h.dinara_batch_set_keep_first(b.ptr, 1),
                                                             h.dinara_batch_set_retry(b.ptr, 1),
                                                             h.dinara_batch_set_persistent(b.ptr, 0)),
+        # The learned path's form: the pairs ready when the retries end are walked while the queue counts come back.
+        "keep first, launches one by one": lambda b: (h.dinara_batch_set_keep_first(b.ptr, 1),
+                                                     h.dinara_batch_set_graph(b.ptr, 0)),
     }
     for name, setup in shapes.items():
         got, _ = results_with(refs, qrys, 1, setup)
         assert got == want, name
+
+
+@pytest.mark.parametrize("keep", [1, 0])
+def test_later_retries_and_a_second_walk(keep):
+    # First bands of no cost, widened by doubling: nearly every pair fails its first band and many fail the doubled
+    # one, so later retries run (a retry by the certified band proves its pair at once). Launched one by one, a run
+    # walks the pairs ready when the retries end while the queue counts come back, then the later retries (and,
+    # without keep_first, the tracing sweeps), then a second walk for what they proved.
+    refs, qrys = uneven(300, 83)
+    want = oracle_alignments(refs, qrys)
+    h = _lib.lib()
+    zeros = np.zeros(len(refs), dtype=np.int64)
+    b = Batch(refs, qrys, 1)
+    try:
+        h.dinara_batch_set_keep_first(b.ptr, keep)
+        h.dinara_batch_set_graph(b.ptr, 0)
+        h.dinara_batch_set_retry(b.ptr, 1)
+        assert h.dinara_batch_seed(b.ptr, zeros.ctypes.data) == 0
+        got = b.run()
+        again = b.run()
+        swept = sum(b.stat(700 + k) for k in range(len(b.widths())))
+    finally:
+        b.free()
+    assert got == want and again == want
+    assert swept > 2 * len(refs), swept  # some pair was swept three times or more: later retries ran
 
 
 def test_keep_first_walks_proven_bands_at_once():
