@@ -12,7 +12,8 @@ pairs, compared byte for byte:
                    bands and launch shape autotuned on the batch, then each proven band traced again;
   learned          the same launches with each pair's first band from a network trained (on the GPU) on the
                    aligner's own distances of separate pairs, seeded on the device inside the step, sweeps keeping
-                   their decisions so a proven pair is walked at once.
+                   their decisions so a proven pair is walked at once; launch form, margin, slope and the first
+                   pass's bin edges (which bins first bands may take) chosen on held-out pairs.
 `value` is the learned path's pairs/s. The restated CPU path (oracle/baseline_edit.cpp, per-pair bit-parallel band
 doubling) is timed on a sample of the pairs on all host cores and checked against the GPU.
 
@@ -723,7 +724,7 @@ def timed_steps(handle, b, steps, warmup, before=None):
     return float(np.mean(walls)), float(np.mean(device))
 
 
-def end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, chosen, floor, use_graph):
+def end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, chosen, floor, use_graph, first_bins):
     """The learned path end to end on a fresh batch, as a caller holding the sequences in host memory meets it: pack
     and upload (dinara_batch_new), learned seeding and the run (the first run of a batch: its graph captured, its
     arena reserved), and the alignments copied out with their CIGARs malloc'ed (dinara_batch_alignments). Then the
@@ -747,6 +748,7 @@ def end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, chosen, 
         raise RuntimeError(f"dinara_batch_new failed: {status.value}")
     t1 = time.perf_counter()
     handle.dinara_batch_set_tuning(b, chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
+    handle.dinara_batch_set_bins(b, first_bins)
     handle.dinara_batch_set_side_by_side(b, chosen[6])
     handle.dinara_batch_set_keep_first(b, 1)
     handle.dinara_batch_set_graph(b, use_graph)
@@ -974,7 +976,7 @@ def run_uneven(args) -> int:
                     print(f"[bench] held-out: margin {mg} slope {sl}: {by_slope[(mg, sl)]:.1f} ms", file=sys.stderr,
                           flush=True)
             margin, slope = min(by_slope, key=by_slope.get)
-        by_floor = {}
+        by_floor, by_bins = {}, {}
         floor = (0, 0)
         if args.floors:
             for item in args.floors.split(","):
@@ -983,6 +985,20 @@ def run_uneven(args) -> int:
                 by_floor[(f3, f4)] = timed_steps(handle, tb, 2, 1, before=seeder(tb, margin, slope))[0]
             floor = min(by_floor, key=by_floor.get)
             handle.dinara_batch_set_tuning(tb, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
+        # The first pass's bin edges: a few bin sets, the bins between the powers of two in or out.
+        widths = [int(handle.dinara_batch_stat(tb, 1200 + k)) for k in range(int(handle.dinara_batch_stat(tb, 12)))]
+        every = (1 << len(widths)) - 1
+        bin_sets = {"every bin": every,
+                    "powers of two": sum(1 << k for k, w in enumerate(widths) if w & (w - 1) == 0),
+                    "every bin but 1344": every & ~(1 << widths.index(1344)),
+                    "every bin but 192 and 1344": every & ~(1 << widths.index(192)) & ~(1 << widths.index(1344))}
+        for name, mask in bin_sets.items():
+            if handle.dinara_batch_set_bins(tb, mask) != 0:
+                raise RuntimeError("bin set refused")
+            by_bins[name] = timed_steps(handle, tb, 3, 1, before=seeder(tb, margin, slope))[0]
+            print(f"[bench] held-out: first bins {name}: {by_bins[name]:.1f} ms", file=sys.stderr, flush=True)
+        bin_set = min(by_bins, key=by_bins.get)
+        first_bins = bin_sets[bin_set]
     handle.dinara_batch_free(tb)
     del tkeep, tune_r, tune_q
 
@@ -1003,6 +1019,7 @@ def run_uneven(args) -> int:
         handle.dinara_batch_set_tuning(batch, learned_chosen[0], (ctypes.c_int64 * 5)(0, 0, 0, *floor))
         handle.dinara_batch_set_side_by_side(batch, learned_chosen[6])
         handle.dinara_batch_set_graph(batch, use_graph)
+        handle.dinara_batch_set_bins(batch, first_bins)
         seed = seeder(batch, margin, slope)
         seed()
         two_pass_ms, _ = timed_steps(handle, batch, args.steps, args.warmup, before=seed)
@@ -1017,7 +1034,7 @@ def run_uneven(args) -> int:
         del keep
         # End to end: packing and upload, seeding, the run, and the alignments copied out (CIGARs malloc'ed).
         end_to_end = end_to_end_learned(handle, refs, qrys, workers, net, margin, slope, learned_chosen, floor,
-                                        use_graph)
+                                        use_graph, first_bins)
         learned_info = {"train_pairs": train_n, "epochs": epochs, "loss": round(loss.value, 4),
                         "label_and_train_s": round(label_train_s, 1), "train_s": round(train_s, 1),
                         "chosen_on": f"{args.tune_pairs or pairs} held-out pairs drawn like the workload (seed 44)",
@@ -1027,6 +1044,9 @@ def run_uneven(args) -> int:
                         "by_launch_ms_held_out": {("graph" if k else "one_by_one"): round(v, 3)
                                                   for k, v in by_launch.items()},
                         "by_floors_ms_held_out": {f"{k[0]}:{k[1]}": round(v, 3) for k, v in by_floor.items()},
+                        "first_pass_bins": [w for k, w in enumerate(widths) if (first_bins >> k) & 1],
+                        "first_pass_bin_set": bin_set,
+                        "by_bin_set_ms_held_out": {k: round(v, 3) for k, v in by_bins.items()},
                         "first_band_proved": round(learned_proven / pairs, 4),
                         "proved_by_a_sweep_keeping_decisions": round(walked_kept / pairs, 4),
                         "two_pass_ms_per_step": round(two_pass_ms, 3), "bins": learned_prof,
@@ -1127,7 +1147,8 @@ def run_uneven(args) -> int:
         "config": {
             "workload": "unit-cost distance + Ties.LEFT CIGAR, 200,000 pairs log-normal around 1.5 kbp (sigma 1, "
                         "100 bp..20 kbp), 1-15% error (50/25/25), learned band seeds + persistent length-binned "
-                        f"launch ({launch_form}), launch shape, margin and slope chosen on held-out pairs",
+                        f"launch ({launch_form}), launch shape, margin, slope and first-pass bin edges chosen on "
+                        "held-out pairs",
             "pairs": pairs,
         },
         "gcups": round(float((n * m).sum()) / (value_ms / 1e3) / 1e9, 2),

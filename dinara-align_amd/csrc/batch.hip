@@ -243,6 +243,7 @@ struct dinara_batch {
     bool timed = true;                    // launches made one by one record an event pair each
     int grid_per_cu = 8;                  // persistent workgroups a compute unit
     int start_bin[EDIT_LENGTH_CLASSES] = {};  // the least first bin, by the pair's longer side
+    uint32_t first_bins = EDIT_ALL_BINS;  // the bins a first band may take (bit b for bin b): the first pass's edges
     std::vector<int32_t> edit_first_low;  // per slot: its first band's lowest diagonal
     std::vector<int32_t> edit_first_bin;
     int64_t first_count[EDIT_BINS] = {};  // pairs each bin's first queue holds
@@ -650,9 +651,9 @@ int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t coun
 
 // ------------------------------------------------------------------ the unit-cost path
 
-// The unit path's first queues: each pair in the bin of its first band, no narrower than its length class's start
-// bin (under band doubling, the first such bin whose width is a power of two), placed around the start and end
-// diagonals. Caller holds the device lock.
+// The unit path's first queues: each pair in the narrowest bin of the first pass's set that holds its first band,
+// no narrower than its length class's start bin (under band doubling, the first such bin of any set whose width is a
+// power of two), placed around the start and end diagonals. Caller holds the device lock.
 int64_t unit_lists(dinara_batch *b) {
     const int64_t slots = (int64_t)b->slot_pair.size();
     const size_t cap = (size_t)std::max<int64_t>(slots, 1);
@@ -664,14 +665,15 @@ int64_t unit_lists(dinara_batch *b) {
         const int n = b->rows[s], m = b->cols[s];
         const int lo = b->first_low[s], hi = b->first_high[s];
         const int from = b->start_bin[edit_length_class(std::max(n, m))];
-        const int bin = b->doubling ? edit_power_bin_for(hi - lo + 1, from) : edit_bin_for(hi - lo + 1, from);
+        // A band placed inside the matrix's diagonals and n + m + 3 wide proves any pair, and no traced band is
+        // wider than n + m + 1: the bins a pair can reach.
+        const int widest = edit_widest(n + m);
+        const int bin = b->doubling ? edit_power_bin_for(hi - lo + 1, from)
+                                    : edit_first_bin(hi - lo + 1, from, b->first_bins, std::max(from, widest));
         if (bin < 0) continue;  // wider than the widest band: left unsettled
         b->edit_first_low[s] = edit_place(lo, hi, edit_width(bin), n, m);
         b->edit_first_bin[s] = bin;
         lists[bin * cap + counts[bin]++] = (int32_t)s;
-        // A band placed inside the matrix's diagonals and n + m + 3 wide proves any pair, and no traced band is
-        // wider than n + m + 1: the bins a pair can reach.
-        const int widest = edit_widest(n + m);
         ++first_count[bin];
         for (int k = bin + 1; k <= widest; ++k) ++retry_reach[k];
         for (int k = 0; k <= widest; ++k) ++trace_reach[k];
@@ -1794,6 +1796,7 @@ int64_t seed_unit_learned(dinara_batch *b, dinara_mlp *net, double margin, doubl
         return code;
     EditStart start{};
     for (int c = 0; c < EDIT_LENGTH_CLASSES; ++c) start.bin[c] = b->start_bin[c];
+    start.bins = b->first_bins;
     launch_edit_seed(edit_args(b), static_cast<const float *>(b->predictions_d.ptr), margin, slope, start,
                      (int)slots, ctx->stream);
     seeded_shape(b);
@@ -1997,6 +2000,17 @@ int64_t dinara_batch_set_tuning(dinara_batch *b, int64_t grid_per_cu, const int6
     return unit_lists(b);
 }
 
+int64_t dinara_batch_set_bins(dinara_batch *b, int64_t bins) {
+    if (!b || bins <= 0 || bins > (int64_t)EDIT_ALL_BINS) return DINARA_INVALID_MODE;
+    b->first_bins = (uint32_t)bins;
+    drop_graph(b);
+    // The host-filed first queues follow the set; seeded ones are filed again by the next seeding.
+    if (!b->unit || b->slot_pair.empty()) return 0;
+    std::lock_guard<std::mutex> serial(b->ctx->lock);
+    DeviceGuard guard(b->ctx);
+    return unit_lists(b);
+}
+
 int64_t dinara_batch_autotune(dinara_batch *b, int64_t *chosen) {
     if (!b) return DINARA_INVALID_MODE;
     return autotune(b, chosen);
@@ -2045,6 +2059,7 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 10: return b->full_pairs;
         case 11: return b->full_ms;
         case 12: return EDIT_BINS;
+        case 13: return b->first_bins;
         default: break;
     }
     if (what >= 600 && what < 1300 && what % 100 < EDIT_BINS) {

@@ -127,6 +127,16 @@ __host__ __device__ inline int edit_widest(int letters) {
     return a > b ? a : b;
 }
 
+// The first pass's bin edges: `bins` (bit b for bin b) is the set of bins a first band may take. A pair's first
+// band takes the narrowest bin of the set that holds `need` diagonals, from `from` on, and no wider than `widest`
+// (the widest bin its launches reach); past it, the narrowest of all bins that holds it. Retries take every bin.
+constexpr uint32_t EDIT_ALL_BINS = (1u << EDIT_BINS) - 1;
+__host__ __device__ inline int edit_first_bin(int need, int from, uint32_t bins, int widest) {
+    for (int bin = from; bin <= widest && bin < EDIT_BINS; ++bin)
+        if (((bins >> bin) & 1u) && need <= edit_width(bin)) return bin;
+    return edit_bin_for(need, from);
+}
+
 // Where a band of `width` diagonals holding [want_low, want_high] starts: centred, kept inside the matrix's
 // diagonals (-rows - 1 .. cols + 1) where it can be.
 __host__ __device__ inline int edit_place(int want_low, int want_high, int width, int rows, int cols) {
@@ -138,9 +148,10 @@ __host__ __device__ inline int edit_place(int want_low, int want_high, int width
     return low;
 }
 
-// Each length class's least first bin.
+// Each length class's least first bin, and the bins first bands may take.
 struct EditStart {
     int bin[EDIT_LENGTH_CLASSES];
+    uint32_t bins;
 };
 
 // Host-side entry points of edit.hip. `grid` is the workgroups of each persistent launch (256 threads each).

@@ -1135,10 +1135,12 @@ __global__ void edit_requeue(EditArgs a, const int32_t *held, int count) {
 }
 
 // Each slot's first band from the network's prediction of its cost (log(1 + cost)): the guess e^(prediction +
-// margin + slope ln(max(longer side, 1000) / 1000)) - 1 (a long pair's failed band costs a long re-sweep), and the band out from the start and end diagonals to the last diagonal a path of the guess's cost
-// could visit (unit costs: a path through diagonal k > max(0, end) costs at least 2k - end, through k < min(0, end)
-// at least end - 2k), one more either side; then the slot is filed in its bin's first queue. A workgroup's slots
-// (consecutive, so of like length) take one place in each queue with one atomic a bin.
+// margin + slope ln(max(longer side, 1000) / 1000)) - 1 (a long pair's failed band costs a long re-sweep), and the
+// band out from the start and end diagonals to the last diagonal a path of the guess's cost could visit (unit costs:
+// a path through diagonal k > max(0, end) costs at least 2k - end, through k < min(0, end) at least end - 2k), one
+// more either side; then the slot is filed in the first queue of the narrowest bin of the first pass's set that
+// holds the band. A workgroup's slots (consecutive, so of like length) take one place in each queue with one atomic
+// a bin.
 __global__ void edit_seed(EditArgs a, const float *prediction, double margin, double slope, EditStart start,
                           int slots) {
     __shared__ int local[EDIT_BINS], base[EDIT_BINS];
@@ -1168,7 +1170,8 @@ __global__ void edit_seed(EditArgs a, const float *prediction, double margin, do
         hi = hi < m + 1 ? hi : m + 1;
         lo = lo > -n - 1 ? lo : -n - 1;
         const int longer = n > m ? n : m;
-        bin = edit_bin_for((int)(hi - lo + 1), start.bin[edit_length_class(longer)]);
+        const int from = start.bin[edit_length_class(longer)], widest = edit_widest(n + m);
+        bin = edit_first_bin((int)(hi - lo + 1), from, start.bins, from > widest ? from : widest);
         if (bin >= 0) {
             low = edit_place((int)lo, (int)hi, edit_width(bin), n, m);
             rank = atomicAdd(&local[bin], 1);

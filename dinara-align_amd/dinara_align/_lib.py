@@ -143,6 +143,8 @@ def lib() -> ctypes.CDLL:
     handle.dinara_batch_set_whole.restype = i64
     handle.dinara_batch_set_tuning.argtypes = [vp, i64, vp]
     handle.dinara_batch_set_tuning.restype = i64
+    handle.dinara_batch_set_bins.argtypes = [vp, i64]
+    handle.dinara_batch_set_bins.restype = i64
     handle.dinara_batch_autotune.argtypes = [vp, vp]
     handle.dinara_batch_autotune.restype = i64
     handle.dinara_mlp_new.argtypes = [i64, ctypes.c_uint64]

@@ -228,7 +228,7 @@ void dinara_batch_free(dinara_batch *batch);
  * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
  * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep), 12 the number of
- * bins; for bin b (widths 64, 128, 192, 256, 384, 512, 768, 1024, 1344, 2048, 4096, 8192, 16384 diagonals) 1200+b
+ * bins, 13 the first pass's bin set (dinara_batch_set_bins); for bin b (widths 64, 128, 192, 256, 384, 512, 768, 1024, 1344, 2048, 4096, 8192, 16384 diagonals) 1200+b
  * its width, 600+b its sweep's time (ms, runs without the graph), 700+b the sweeps it began, 800+b its tracing
  * sweep's time, 900+b the traces it began, 1000+b and 1100+b the word-columns (a pair's columns times its band's
  * 64-bit words) its sweeps and traces covered. Every
@@ -297,6 +297,12 @@ int64_t dinara_batch_set_whole(dinara_batch *batch, int64_t on);
  * (below 256, 1024, 4096, 16384 letters, and longer) the least bin its first band takes (0..12, in the widths' order
  * given for dinara_batch_stat); null keeps them. */
 int64_t dinara_batch_set_tuning(dinara_batch *batch, int64_t grid_per_cu, const int64_t *start_bins);
+/* The first pass's bin edges: the bins a first band may take, bit b for bin b (in the widths' order given for
+ * dinara_batch_stat; every bin, 8191, by default). A pair's first band takes the narrowest bin of the set that holds
+ * it and that its launches reach, else the narrowest bin of all that holds it; retries take every bin, and plain
+ * band doubling keeps to the powers of two. DINARA_INVALID_MODE for an empty set or bits past bin 12. Results do
+ * not change, only the work. */
+int64_t dinara_batch_set_bins(dinara_batch *batch, int64_t bins);
 /* Each bin's first queue, and each bin's tracing sweep, on a stream of its own so the bins run side by side (1, the
  * default), or the bins one after another (0). */
 int64_t dinara_batch_set_side_by_side(dinara_batch *batch, int64_t on);

@@ -160,6 +160,48 @@ def test_bins_between_the_powers_of_two(right, keep):
         b.free()
 
 
+@pytest.mark.parametrize("keep", [1, 0])
+def test_first_pass_bin_sets(keep):
+    # The first pass's bin edges: host seeds above every pair's cost (so every first band proves its pair) that ask
+    # for bands of 192, 384, 768 and 1344 diagonals, filed under several bin sets. With the powers of two only,
+    # every first band takes 256, 512, 1024 or 2048 diagonals and the bins between sweep nothing; a set with no bin
+    # wide enough falls back to the narrowest of all; device seeding follows the set too. No result changes.
+    rng = random.Random(91 + keep)
+    refs, qrys = pairs_of(48, rng, (1000, 1800), (0.02, 0.05))
+    want = oracle_alignments(refs, qrys)
+    guesses = np.array([150, 300, 600, 1100] * 12, dtype=np.int64)
+    assert all(cost <= guess for (cost, _), guess in zip(want, guesses))
+    h = _lib.lib()
+    b = Batch(refs, qrys, 1)
+    try:
+        widths = b.widths()
+        powers = sum(1 << k for k, w in enumerate(widths) if w & (w - 1) == 0)
+        assert h.dinara_batch_set_bins(b.ptr, 0) == _lib.INVALID_MODE
+        assert h.dinara_batch_set_bins(b.ptr, 1 << len(widths)) == _lib.INVALID_MODE
+        assert int(b.stat(13)) == (1 << len(widths)) - 1
+        h.dinara_batch_set_keep_first(b.ptr, keep)
+        for bins in (powers, 1, (1 << len(widths)) - 1, sum(1 << k for k, w in enumerate(widths) if w in (512, 4096))):
+            assert h.dinara_batch_set_bins(b.ptr, bins) == 0
+            assert int(b.stat(13)) == bins
+            assert h.dinara_batch_seed(b.ptr, guesses.ctypes.data) == 0
+            assert b.run() == want, bins
+            swept = [b.stat(700 + k) for k in range(len(widths))]
+            assert sum(swept) == len(refs), (bins, swept)  # every first band proved its pair
+            if bins == powers:
+                assert all(swept[k] == 0 for k, w in enumerate(widths) if w & (w - 1)), swept
+                assert all(swept[widths.index(w)] > 0 for w in (256, 512, 1024, 2048)), swept
+        net = h.dinara_mlp_new(0, 5)
+        try:
+            for bins in (powers, (1 << len(widths)) - 1):
+                assert h.dinara_batch_set_bins(b.ptr, bins) == 0
+                assert h.dinara_batch_seed_learned(b.ptr, net, ctypes.c_double(0.0)) == 0
+                assert b.run() == want, bins
+        finally:
+            h.dinara_mlp_free(net)
+    finally:
+        b.free()
+
+
 def test_words_of_two_and_four_a_lane():
     # Unrelated sequences of 5,000..9,000 letters: distances near half their length need bands of 8192 and 16384
     # diagonals (64 lanes of two and four words).
