@@ -197,6 +197,7 @@ struct dinara_batch {
     int group_letters = 4;                // LDS bytes a pair takes in a group launch
     int group_max_letters = 0;            // the longest pair a group launch stages
     bool groups_fit = false;              // four pairs' letters fit one block's LDS, their costs the tags
+    int group16_top = 0;                  // > 0: packed 16-bit group cells, this the unreachable value
     int block_mask = 0;                   // the block classes whose letters and tags fit
     int64_t dearest = 1;                  // the dearest single step: a substitution or a gap's first letter
     // Free ends (wave sweeps): the letters each end of each sequence may leave out (reference start, end, query
@@ -442,6 +443,21 @@ int64_t prepare(dinara_batch *b, const char *const *refs, const char *const *qry
     // The group and block sweeps hold one gap piece: two-piece batches take the wave sweeps alone.
     b->groups_fit = !b->parsed.two_piece && !b->free_ends && group_cap > 0 && (size_t)4 * b->group_letters <= b->lds_limit &&
                     ((int64_t)b->group_max_letters + 2 * 256 + 8) * b->dearest < REAL_LIMIT;
+    // Packed 16-bit group cells (group.hip) for shifted costs (equal extensions e) whose mismatch is no dearer than
+    // two extension letters: a tagged value then lies in [-16 e (n + m), 32 opening + 15] and a cell no path
+    // reaches stays above `top` - 16 e (n + m), so both fit 16 bits, apart, while 16 e (n + m) + 32 opening + 15
+    // stays under `top` (less a margin); `top` leaves room for an opening's add. DINARA_GROUP_CELLS32 set keeps
+    // 32-bit cells, for comparison.
+    b->group16_top = 0;
+    {
+        const CostParams &k = b->costs;
+        const int64_t e = k.del_ext, opening = std::max(k.del_open, k.ins_open);
+        const int64_t top = (32767 - 16 * opening - 32) & ~int64_t(15);
+        if (b->groups_fit && !std::getenv("DINARA_GROUP_CELLS32") && k.pieces != 2 && k.del_ext == k.ins_ext &&
+            e >= 0 && k.mismatch >= 0 && k.mismatch <= 2 * e && std::min(k.del_open, k.ins_open) >= 0 &&
+            16 * e * b->group_max_letters + 32 * opening + 15 < top - 64 && (size_t)8 * b->group_letters <= b->lds_limit)
+            b->group16_top = (int)top;
+    }
     b->block_mask = 0;
     for (int cls = CLASS_BLOCK_FIRST; cls < CLASSES && !b->parsed.two_piece && !b->free_ends; ++cls)
         if (block_letters(b->max_letters, cls) <= b->lds_limit &&
@@ -578,6 +594,7 @@ SweepArgs sweep_args(dinara_batch *b) {
     a.doubling = b->doubling ? 1 : 0;
     a.groups_fit = b->groups_fit;
     a.group_max_letters = b->group_max_letters;
+    a.group16_top = b->group16_top;
     a.block_mask = b->block_mask;
     a.cigar_scratch = static_cast<char *>(b->cigar_scratch.ptr);
     a.cigar_scratch_base = static_cast<const int64_t *>(b->cigar_scratch_base.ptr);
@@ -619,7 +636,7 @@ int64_t launch_class(dinara_batch *b, int cls, const int32_t *list, int64_t coun
     const bool wave = class_is_wave(cls);
     const size_t lds = wave                     ? (size_t)std::min<int>(std::max(b->max_letters, 4), (int)b->lds_limit)
                        : band_is_block(cls) ? block_letters(b->max_letters, cls)
-                                            : (size_t)4 * b->group_letters;
+                                            : (size_t)(b->group16_top > 0 && cls >= 1 ? 8 : 4) * b->group_letters;
     int64_t chunk = count;
     size_t stride = 0;
     if (a.record) {
@@ -2060,6 +2077,7 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
         case 11: return b->full_ms;
         case 12: return EDIT_BINS;
         case 13: return b->first_bins;
+        case 14: return b->group16_top;
         default: break;
     }
     if (what >= 600 && what < 1300 && what % 100 < EDIT_BINS) {

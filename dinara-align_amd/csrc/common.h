@@ -113,6 +113,8 @@ struct SweepArgs {
     int group_letters;       // LDS bytes a pair takes in a group launch (four pairs a block)
     int groups_fit;          // 1 when four pairs' letters fit one block's LDS
     int group_max_letters;   // the longest pair (both sequences) a group launch stages; longer pairs take blocks
+    int group16_top;         // > 0: group launches of two slots a lane and more hold their steady cells as 16-bit
+                             // halves, `group16_top` the unreachable value (letters then take 2 bytes each in LDS)
     int block_mask;          // bit c - CLASS_BLOCK_FIRST: block class c's letters fit its LDS and its costs its tags
     // CIGARs: each slot's scratch run of 2(n+m)+2 bytes from cigar_scratch_base, compacted into `cigars` at an
     // offset taken from `cigar_used`

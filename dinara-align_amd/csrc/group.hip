@@ -109,13 +109,14 @@ __device__ __forceinline__ void group_cell(const GroupCosts &g, bool eq, int hd,
 }
 
 // The careful form: row 0 is an insertion of every column before, column 0 a deletion of every row, outside the
-// matrix unreachable; the letters are read clamped inside both sequences. Also returns the tagged H in `h`.
-template <bool RECORD, bool SH>
+// matrix unreachable; the letters are read clamped inside both sequences, STRIDE bytes apart. Also returns the
+// tagged H in `h`.
+template <bool RECORD, bool SH, int STRIDE = 1>
 __device__ __forceinline__ void careful_cell(const GroupCosts &g, int r, int c, const uint8_t *ref, const uint8_t *qry,
                                              int last_ref, int last_qry, int hd, int dx, int ix, int &h, int &hc,
                                              int &dn, int &in, uint32_t &acc) {
     const int ri = min(max(r - 1, 0), last_ref), ci = min(max(c - 1, 0), last_qry);
-    const bool eq = ref[ri] == qry[ci];
+    const bool eq = ref[STRIDE * ri] == qry[STRIDE * ci];
     const int sub = hd + (eq ? g.sub_eq : g.sub_ne);
     const int dc = RECORD ? (dx & ~2) : dx;
     const int ic = RECORD ? (ix & ~1) : ix;
@@ -144,7 +145,7 @@ struct GroupState {
 // Even anti-diagonal: even slot s's cell takes D from odd slot s (one diagonal up) and I from odd slot s - 1
 // (the lane below's last for slot 0, in from_lower_i, which the caller has shifted in). The careful form captures
 // the corner's tagged H when it is this cell.
-template <int S, bool RECORD, bool SH, bool CAREFUL>
+template <int S, bool RECORD, bool SH, bool CAREFUL, int STRIDE = 1>
 __device__ __forceinline__ void even_half(GroupState<S> &st, const GroupCosts &g, int t, int base_k,
                                           const uint8_t *ref, const uint8_t *qry, int last_ref, int last_qry,
                                           const uint8_t *rp, const uint8_t *qp, int first_q, int q_end, int tend,
@@ -156,8 +157,8 @@ __device__ __forceinline__ void even_half(GroupState<S> &st, const GroupCosts &g
         if (CAREFUL) {
             const int kk = base_k + 2 * s, r = (t - kk) >> 1;
             int h;
-            careful_cell<RECORD, SH>(g, r, r + kk, ref, qry, last_ref, last_qry, st.He[s], st.Do[s], ix, h, hc, dn,
-                                     in, st.acc[s]);
+            careful_cell<RECORD, SH, STRIDE>(g, r, r + kk, ref, qry, last_ref, last_qry, st.He[s], st.Do[s], ix, h,
+                                             hc, dn, in, st.acc[s]);
             if (t == tend && first_q + s == q_end) corner = h;
         } else {
             group_cell<RECORD, SH>(g, rp[-s] == qp[s], st.He[s], st.Do[s], ix, hc, dn, in, st.acc[s]);
@@ -170,7 +171,7 @@ __device__ __forceinline__ void even_half(GroupState<S> &st, const GroupCosts &g
 
 // Odd anti-diagonal: odd slot s's cell takes D from even slot s + 1 (the lane above's first for the last slot, in
 // from_upper_d, which the caller has shifted in) and I from even slot s.
-template <int S, bool RECORD, bool SH, bool CAREFUL>
+template <int S, bool RECORD, bool SH, bool CAREFUL, int STRIDE = 1>
 __device__ __forceinline__ void odd_half(GroupState<S> &st, const GroupCosts &g, int t, int base_k,
                                          const uint8_t *ref, const uint8_t *qry, int last_ref, int last_qry,
                                          const uint8_t *rp, const uint8_t *qp, int first_q, int q_end, int tend,
@@ -182,8 +183,8 @@ __device__ __forceinline__ void odd_half(GroupState<S> &st, const GroupCosts &g,
         if (CAREFUL) {
             const int kk = base_k + 2 * s + 1, r = (t - kk) >> 1;
             int h;
-            careful_cell<RECORD, SH>(g, r, r + kk, ref, qry, last_ref, last_qry, st.Ho[s], dx, st.Ie[s], h, hc, dn,
-                                     in, st.acc[s]);
+            careful_cell<RECORD, SH, STRIDE>(g, r, r + kk, ref, qry, last_ref, last_qry, st.Ho[s], dx, st.Ie[s], h,
+                                             hc, dn, in, st.acc[s]);
             if (t == tend && first_q + s == q_end) corner = h;
         } else {
             group_cell<RECORD, SH>(g, rp[-s] == qp[s + 1], st.Ho[s], dx, st.Ie[s], hc, dn, in, st.acc[s]);
@@ -194,10 +195,137 @@ __device__ __forceinline__ void odd_half(GroupState<S> &st, const GroupCosts &g,
     }
 }
 
-template <int S, bool RECORD, bool SH>
+// Packed cells: two slots of a lane in the 16-bit halves of one register, slot q low and slot q + S/2 high, for
+// the steady anti-diagonals of a shifted sweep (both extensions equal). There a tagged value is 16 (H - e(r + c))
+// plus its tag, between -16 e (r + c) and 32 max(opening) + 15, and a cell no path inside the band reaches holds
+// `top` less at most 16 e (r + c): the host takes this form only where both fit 16 bits apart (group16_top), so
+// every min picks what the 32-bit form picks and the low nibble of every sum, the decision, is the same. The adds
+// wrap as 16-bit integers and never reach the wrap; the mins are signed.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_min(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_min(__builtin_bit_cast(i16x2, a), __builtin_bit_cast(i16x2, b)));
+}
+__device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) * __builtin_bit_cast(u16x2, b) +
+                                            __builtin_bit_cast(u16x2, c));
+}
+__device__ __forceinline__ uint32_t both_halves(int v) { return ((uint32_t)v & 0xFFFFu) * 0x10001u; }
+__device__ __forceinline__ uint32_t pack_pair(int lo, int hi, int top) {
+    return ((uint32_t)min(lo, top) & 0xFFFFu) | ((uint32_t)min(hi, top) << 16);
+}
+
+struct PackedCosts {
+    uint32_t sub_eq, sub_gap, open_d, open_i;  // both halves: two equal letters, unequal less equal, the openings
+};
+
+template <int S>
+struct PackedState {
+    uint32_t He[S / 2], De[S / 2], Ie[S / 2], Ho[S / 2], Do[S / 2], Io[S / 2];
+    uint32_t lower, upper;  // the DPP receivers; an edge lane's keeps `top`
+};
+
+// One register of cells: `apart` the two slots' letter differences in bytes 0 and 1 (zero: equal letters).
+template <bool RECORD>
+__device__ __forceinline__ void packed_cell(const PackedCosts &k, uint32_t apart, uint32_t hd, uint32_t dx,
+                                            uint32_t ix, uint32_t &hc, uint32_t &dn, uint32_t &in, uint32_t &acc_lo,
+                                            uint32_t &acc_hi) {
+    const uint32_t unequal = pk_min(__builtin_amdgcn_perm(0u, apart, 0x0C010C00u), 0x00010001u);  // bytes to halves
+    const uint32_t sub = pk_add(hd, pk_mad(unequal, k.sub_gap, k.sub_eq));
+    const uint32_t dc = RECORD ? (dx & 0xFFFDFFFDu) : dx;
+    const uint32_t ic = RECORD ? (ix & 0xFFFEFFFEu) : ix;
+    const uint32_t h = pk_min(sub, pk_min(dc, ic));
+    hc = RECORD ? (h & 0xFFF0FFF0u) : h;
+    dn = pk_min(pk_add(hc, k.open_d), dc);
+    in = pk_min(pk_add(hc, k.open_i), ic);
+    if (RECORD) {
+        const uint32_t y = pk_add(dn ^ in, h);  // each half's low nibble: the 32-bit form's decision
+        acc_lo = __builtin_amdgcn_alignbit(y, acc_lo, 4);
+        acc_hi = __builtin_amdgcn_alignbit(y >> 16, acc_hi, 4);
+    }
+}
+
+// The packed half-steps. Letters: `rp` the reference's pairs at r0 - 1 (byte 0 letter i, byte 1 letter i - S/2),
+// `qp` the query's at c0 - 1 (byte 0 letter i, byte 1 letter i + S/2). Even slot pair q takes I from odd pair q - 1
+// (pair 0: the lane below's last slot and its own slot S/2 - 1); odd pair q takes D from even pair q + 1 (the last:
+// its own slot S/2 and the lane above's slot 0).
+template <int S, bool RECORD>
+__device__ __forceinline__ void packed_even(PackedState<S> &ps, GroupState<S> &st, const PackedCosts &k,
+                                            const uint16_t *rp, const uint16_t *qp) {
+    constexpr int P = S / 2;
+    ps.lower = (uint32_t)shift_from_lower<false>((int)ps.lower, (int)ps.Io[P - 1]);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const uint32_t ix = q > 0 ? ps.Io[q > 0 ? q - 1 : 0] : __builtin_amdgcn_alignbit(ps.Io[P - 1], ps.lower, 16);
+        uint32_t hc, dn, in;
+        packed_cell<RECORD>(k, (uint32_t)rp[-q] ^ (uint32_t)qp[q], ps.He[q], ps.Do[q], ix, hc, dn, in, st.acc[q],
+                            st.acc[q + P]);
+        ps.He[q] = hc;
+        ps.De[q] = dn;
+        ps.Ie[q] = in;
+    }
+}
+
+template <int S, bool RECORD>
+__device__ __forceinline__ void packed_odd(PackedState<S> &ps, GroupState<S> &st, const PackedCosts &k,
+                                           const uint16_t *rp, const uint16_t *qp) {
+    constexpr int P = S / 2;
+    ps.upper = (uint32_t)shift_from_upper<false>((int)ps.upper, (int)ps.De[0]);
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const uint32_t dx = q + 1 < P ? ps.De[q + 1 < P ? q + 1 : 0] : __builtin_amdgcn_alignbit(ps.upper, ps.De[0], 16);
+        uint32_t hc, dn, in;
+        packed_cell<RECORD>(k, (uint32_t)rp[-q] ^ (uint32_t)qp[q + 1], ps.Ho[q], dx, ps.Ie[q], hc, dn, in, st.acc[q],
+                            st.acc[q + P]);
+        ps.Ho[q] = hc;
+        ps.Do[q] = dn;
+        ps.Io[q] = in;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void pack_state(PackedState<S> &ps, const GroupState<S> &st, int top) {
+    constexpr int P = S / 2;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        ps.He[q] = pack_pair(st.He[q], st.He[q + P], top);
+        ps.De[q] = pack_pair(st.De[q], st.De[q + P], top);
+        ps.Ie[q] = pack_pair(st.Ie[q], st.Ie[q + P], top);
+        ps.Ho[q] = pack_pair(st.Ho[q], st.Ho[q + P], top);
+        ps.Do[q] = pack_pair(st.Do[q], st.Do[q + P], top);
+        ps.Io[q] = pack_pair(st.Io[q], st.Io[q + P], top);
+    }
+    ps.lower = ps.upper = pack_pair(top, top, top);
+}
+
+template <int S>
+__device__ __forceinline__ void unpack_state(GroupState<S> &st, const PackedState<S> &ps) {
+    constexpr int P = S / 2;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        st.He[q] = __builtin_amdgcn_sbfe((int)ps.He[q], 0, 16), st.He[q + P] = (int)ps.He[q] >> 16;
+        st.De[q] = __builtin_amdgcn_sbfe((int)ps.De[q], 0, 16), st.De[q + P] = (int)ps.De[q] >> 16;
+        st.Ie[q] = __builtin_amdgcn_sbfe((int)ps.Ie[q], 0, 16), st.Ie[q + P] = (int)ps.Ie[q] >> 16;
+        st.Ho[q] = __builtin_amdgcn_sbfe((int)ps.Ho[q], 0, 16), st.Ho[q + P] = (int)ps.Ho[q] >> 16;
+        st.Do[q] = __builtin_amdgcn_sbfe((int)ps.Do[q], 0, 16), st.Do[q + P] = (int)ps.Do[q] >> 16;
+        st.Io[q] = __builtin_amdgcn_sbfe((int)ps.Io[q], 0, 16), st.Io[q + P] = (int)ps.Io[q] >> 16;
+    }
+}
+
+// PACK: the steady anti-diagonals in packed cells (S >= 2, shifted). A pair's letters are then staged as 16-bit
+// positions, the reference's position i holding letters i and i - S/2, the query's i and i + S/2 (the careful
+// cells read the low bytes).
+template <int S, bool RECORD, bool SH, bool PACK>
 __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
+    static_assert(!PACK || (SH && S >= 2 && S % 2 == 0), "packed cells: shifted sweeps of two slots or more");
     extern __shared__ uint8_t letters[];
     constexpr int W = 32 * S;
+    constexpr int STRIDE = PACK ? 2 : 1;  // bytes between a sequence's letters in LDS
+    constexpr int P = S / 2;
     const int lane = threadIdx.x & 15;
     const int local = threadIdx.x >> 4;  // the pair within the block (four a block)
     const int index = blockIdx.x * 4 + local;
@@ -212,19 +340,26 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     const GroupCosts g = group_costs<RECORD, SH>(a.costs);
 
     // A pair's region: its reference, its query, and room past the query for the steady state's reads, which run
-    // up to half a band past either sequence's end by the pair's last anti-diagonal.
-    uint8_t *ref = letters + local * a.group_letters;
-    uint8_t *qry = ref + n;
+    // up to half a band past either sequence's end by the pair's last anti-diagonal. Packed: the reference's S/2
+    // positions past its end hold its last letters' second bytes.
+    uint8_t *ref = letters + (size_t)local * a.group_letters * STRIDE;
+    uint8_t *qry = ref + STRIDE * (n + (PACK ? P : 0));
     {
         const int group = slot >> 6, column = slot & 63;
         const uint32_t *ref_words = a.tape.words + a.tape.group_base[group] + column;
         const uint32_t *qry_words = ref_words + (size_t)a.tape.group_ref_words[group] * 64;
         const int bits = a.tape.bits, per = 32 / bits;
         const uint32_t mask = (1u << bits) - 1;
-        for (int i = lane; i < n; i += 16)
-            ref[i] = (uint8_t)((ref_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
-        for (int i = lane; i < m; i += 16)
-            qry[i] = (uint8_t)((qry_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+        for (int i = lane; i < n; i += 16) {
+            const uint8_t letter = (uint8_t)((ref_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+            ref[STRIDE * i] = letter;
+            if (PACK) ref[STRIDE * (i + P) + 1] = letter;
+        }
+        for (int i = lane; i < m; i += 16) {
+            const uint8_t letter = (uint8_t)((qry_words[(size_t)(i / per) * 64] >> (bits * (i % per))) & mask);
+            qry[STRIDE * i] = letter;
+            if (PACK && i >= P) qry[STRIDE * (i - P) + 1] = letter;
+        }
     }
     __syncthreads();
 
@@ -285,11 +420,11 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     };
     auto careful_pair = [&](int t) __attribute__((always_inline)) {
         st.from_lower_i = shift_from_lower<false>(st.from_lower_i, st.Io[S - 1]);
-        even_half<S, RECORD, SH, true>(st, g, t, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
-                                       q_end, tend, corner);
+        even_half<S, RECORD, SH, true, STRIDE>(st, g, t, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr,
+                                               first_q, q_end, tend, corner);
         st.from_upper_d = shift_from_upper<false>(st.from_upper_d, st.De[0]);
-        odd_half<S, RECORD, SH, true>(st, g, t + 1, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr, first_q,
-                                      q_end, tend, corner);
+        odd_half<S, RECORD, SH, true, STRIDE>(st, g, t + 1, base_k, ref, qry, last_ref, last_qry, nullptr, nullptr,
+                                              first_q, q_end, tend, corner);
         if (((t + 1) & 7) == 7) finish_block(t + 1);
     };
     int t = 0;
@@ -300,16 +435,32 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
     // four blocks at a time, their blocks' places known.
     if (t <= tmax) {
         int r0 = (t - base_k) >> 1, c0 = r0 + base_k;
+        PackedState<S> ps;
+        PackedCosts pk{};
+        if constexpr (PACK) {
+            pk.sub_eq = both_halves(g.sub_eq);
+            pk.sub_gap = both_halves(g.sub_ne - g.sub_eq);
+            pk.open_d = both_halves(g.open_d);
+            pk.open_i = both_halves(g.open_i);
+            pack_state<S>(ps, st, a.group16_top);
+        }
         auto steady_block = [&]() __attribute__((always_inline)) {
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                const uint8_t *rp = ref + r0 - 1, *qp = qry + c0 - 1;
-                st.from_lower_i = shift_from_lower<false>(st.from_lower_i, st.Io[S - 1]);
-                even_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
-                                                q_end, tend, corner);
-                st.from_upper_d = shift_from_upper<false>(st.from_upper_d, st.De[0]);
-                odd_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
-                                               q_end, tend, corner);
+                if constexpr (PACK) {
+                    const uint16_t *rp = reinterpret_cast<const uint16_t *>(ref) + r0 - 1;
+                    const uint16_t *qp = reinterpret_cast<const uint16_t *>(qry) + c0 - 1;
+                    packed_even<S, RECORD>(ps, st, pk, rp, qp);
+                    packed_odd<S, RECORD>(ps, st, pk, rp, qp);
+                } else {
+                    const uint8_t *rp = ref + r0 - 1, *qp = qry + c0 - 1;
+                    st.from_lower_i = shift_from_lower<false>(st.from_lower_i, st.Io[S - 1]);
+                    even_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
+                                                    q_end, tend, corner);
+                    st.from_upper_d = shift_from_upper<false>(st.from_upper_d, st.De[0]);
+                    odd_half<S, RECORD, SH, false>(st, g, t, base_k, ref, qry, last_ref, last_qry, rp, qp, first_q,
+                                                   q_end, tend, corner);
+                }
                 ++r0;
                 ++c0;
             }
@@ -332,6 +483,7 @@ __global__ __launch_bounds__(64) void sweep_group(SweepArgs a) {
             steady_block();
             finish_block(t + 7);
         }
+        if constexpr (PACK) unpack_state<S>(st, ps);
     }
     // The tail: every pair's corner, with the borders' care.
     for (; t <= tmax; t += 2) careful_pair(t);
@@ -561,16 +713,20 @@ template <int S>
 static void launch_group_s(const SweepArgs &a, size_t lds, hipStream_t stream) {
     const dim3 grid((a.count + 3) / 4), block(64);
     const bool shifted = a.costs.del_ext == a.costs.ins_ext;
+    const bool packed = S >= 2 && shifted && a.group16_top > 0;
     if (a.record) {
-        if (shifted) hipLaunchKernelGGL((sweep_group<S, true, true>), grid, block, lds, stream, a);
-        else hipLaunchKernelGGL((sweep_group<S, true, false>), grid, block, lds, stream, a);
+        if (packed) hipLaunchKernelGGL((sweep_group<S, true, true, S >= 2>), grid, block, lds, stream, a);
+        else if (shifted) hipLaunchKernelGGL((sweep_group<S, true, true, false>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_group<S, true, false, false>), grid, block, lds, stream, a);
     } else {
-        if (shifted) hipLaunchKernelGGL((sweep_group<S, false, true>), grid, block, lds, stream, a);
-        else hipLaunchKernelGGL((sweep_group<S, false, false>), grid, block, lds, stream, a);
+        if (packed) hipLaunchKernelGGL((sweep_group<S, false, true, S >= 2>), grid, block, lds, stream, a);
+        else if (shifted) hipLaunchKernelGGL((sweep_group<S, false, true, false>), grid, block, lds, stream, a);
+        else hipLaunchKernelGGL((sweep_group<S, false, false, false>), grid, block, lds, stream, a);
     }
 }
 
-// Launches a group class: S = 1, 2, 4 or 8 (32 to 256 diagonals); `lds` four pairs' letters.
+// Launches a group class: S = 1, 2, 4 or 8 (32 to 256 diagonals); `lds` four pairs' letters (16-bit positions when
+// the batch's group16_top is set and S >= 2).
 void launch_group(int s, const SweepArgs &a, size_t lds, hipStream_t stream) {
     switch (s) {
         case 1: launch_group_s<1>(a, lds, stream); break;

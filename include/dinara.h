@@ -228,7 +228,9 @@ void dinara_batch_free(dinara_batch *batch);
  * 1024, 12288, 16384; the last two distances only). Unit-cost batches (the bit-parallel path): 6 is 1 when the
  * batch took it, 7 is 1 when the run replayed its captured hipGraph, 8 the further tracing passes the arena
  * needed, 9 the pairs proven by a sweep that kept its decisions (walked without a second sweep), 12 the number of
- * bins, 13 the first pass's bin set (dinara_batch_set_bins); for bin b (widths 64, 128, 192, 256, 384, 512, 768, 1024, 1344, 2048, 4096, 8192, 16384 diagonals) 1200+b
+ * bins, 13 the first pass's bin set (dinara_batch_set_bins); 14 above zero when the gap-cost group sweeps hold
+ * packed 16-bit cells (shifted costs, mismatch at most two extensions, pairs short enough), 0 for 32-bit ones; for
+ * bin b (widths 64, 128, 192, 256, 384, 512, 768, 1024, 1344, 2048, 4096, 8192, 16384 diagonals) 1200+b
  * its width, 600+b its sweep's time (ms, runs without the graph), 700+b the sweeps it began, 800+b its tracing
  * sweep's time, 900+b the traces it began, 1000+b and 1100+b the word-columns (a pair's columns times its band's
  * 64-bit words) its sweeps and traces covered. Every

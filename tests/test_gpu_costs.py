@@ -197,6 +197,67 @@ def class_pairs(refs, qrys, costs, kind):
         handle.dinara_batch_free(batch)
 
 
+def resident_alignments(refs, qrys, costs):
+    """A resident batch's alignments through the C ABI, and stat 14 (above zero: packed 16-bit group cells)."""
+    from dinara_align import _lib
+    handle = _lib.lib()
+    R, Q = _lib.Texts(refs), _lib.Texts(qrys)
+    c = costs.as_c()
+    status = ctypes.c_int64(0)
+    batch = handle.dinara_batch_new(0, _lib.BATCH_ALIGNMENTS, len(refs), *R.args(), *Q.args(), ctypes.byref(c), None,
+                                    None, 4, ctypes.byref(status))
+    assert batch, status.value
+    try:
+        assert handle.dinara_batch_run(batch) == 0
+        found = (_lib.CAlignment * len(refs))()
+        statuses = (ctypes.c_int64 * len(refs))()
+        assert handle.dinara_batch_alignments(batch, found, statuses) == 0
+        out = []
+        for i in range(len(refs)):
+            assert statuses[i] == 0, (i, statuses[i])
+            out.append((found[i].cost, ctypes.string_at(found[i].cigar, found[i].cigar_length).decode()))
+            handle.dinara_free(found[i].cigar)
+        return out, handle.dinara_batch_stat(batch, 14)
+    finally:
+        handle.dinara_batch_free(batch)
+
+
+@pytest.mark.parametrize("costs", [da.Costs.affine(4, 6, 2), da.Costs.affine(2, 3, 1), da.Costs.affine(1, 5, 3)])
+def test_packed_group_cells(costs, monkeypatch):
+    # Shifted costs whose mismatch is at most two extension letters: the group sweeps hold two slots a register as
+    # 16-bit halves while the batch's longest group pair stays under a bound (about 1,000 letters at (4, 6, 2)).
+    # Pairs up to that bound, tie-rich repeats, unrelated and ragged pairs, against the oracle and the 32-bit cells;
+    # a batch past the bound takes the 32-bit cells.
+    rng = random.Random(costs.mismatch * 100 + costs.opening * 10 + costs.extension)
+    limit = {(4, 6, 2): 1000, (2, 3, 1): 2000, (1, 5, 3): 640}[(costs.mismatch, costs.opening, costs.extension)]
+    refs, qrys = random_pairs(60, rng, lengths=(limit // 2 - 60, limit // 2 - 30), rates=(0.01, 0.05, 0.15, 0.3))
+    for unit in ("A", "AC", "ACG", "AACC"):
+        r = (unit * limit)[: limit // 2 - 20]
+        refs.append(r)
+        qrys.append(mutated(r, 0.1, rng))
+    for _ in range(8):
+        refs.append("".join(rng.choice("ACGT") for _ in range(rng.randint(limit // 2 - 100, limit // 2 - 20))))
+        qrys.append("".join(rng.choice("ACGT") for _ in range(rng.randint(limit // 2 - 100, limit // 2 - 20))))
+    refs += ["ACGT" * 3, "", "A" * (limit // 2 - 10)]
+    qrys += ["AGT", "ACG", "A" * (limit // 2 - 40)]
+    assert max(len(r) + len(q) for r, q in zip(refs, qrys)) <= limit
+    oc = to_oracle(costs)
+    want = [o.costs_align(r, q, oc) for r, q in zip(refs, qrys)]
+    packed, top = resident_alignments(refs, qrys, costs)
+    assert top > 0
+    assert packed == want
+    monkeypatch.setenv("DINARA_GROUP_CELLS32", "1")
+    wide, top = resident_alignments(refs, qrys, costs)
+    assert top == 0 and wide == want
+    monkeypatch.delenv("DINARA_GROUP_CELLS32")
+    # Past the bound: one pair of 60 letters more than it.
+    refs.append("ACGT" * ((limit + 60) // 8))
+    qrys.append(mutated(refs[-1], 0.05, rng))
+    beyond, top = resident_alignments(refs, qrys, costs)
+    assert top == 0
+    assert beyond[:-1] == want and beyond[-1] == o.costs_align(refs[-1], qrys[-1], oc)
+
+
 def test_block_bands_one_and_several_waves():
     # Long, noisy pairs: bands of 512 to 8192 diagonals, one pair a block of one to eight waves.
     rng = random.Random(37)
