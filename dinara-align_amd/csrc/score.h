@@ -39,6 +39,8 @@ struct ScoreArgs {
     int letters;
     int open, extend;        // a gap's first letter, each later one (scores, zero or less)
     int mode;
+    int end_bits;            // SCORE_LOCAL: > 0 keys each column's best as score << end_bits | row (rows and scores
+                             // fit the 32 bits), 0 compares and selects
     int c;                   // columns a lane
     int32_t *score;          // per slot
     int32_t *end_row;        // SCORE_LOCAL: the best cell's row and column; SCORE_REACH: the found cell's

@@ -48,7 +48,10 @@ __device__ __forceinline__ bool earlier_start(int r, int c, int br, int bc) {
     return r < br;
 }
 
-template <int C, int MODE>
+// KEYED (SCORE_LOCAL): a column's best is one unsigned key, score << end_bits | row, so one max a cell keeps the
+// highest score and, among equal scores, the latest row, as the compare-and-select form does (the host picks it
+// where every score and row fit the key).
+template <int C, int MODE, bool KEYED = false>
 __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
     constexpr bool LOCAL = MODE == SCORE_LOCAL;
     constexpr bool REACH = MODE == SCORE_REACH;
@@ -175,7 +178,9 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
                                              (il + a.extend > hl + a.open ? 8u : 0u);
                         word |= nib << (4 * i);
                     }
-                    if (LOCAL && h >= col_h[i] && h > 0) {
+                    if constexpr (LOCAL && KEYED) {
+                        col_h[i] = (int)max((uint32_t)col_h[i], ((uint32_t)h << a.end_bits) | (uint32_t)r);
+                    } else if (LOCAL && h >= col_h[i] && h > 0) {
                         col_h[i] = h;
                         col_r[i] = r;
                     }
@@ -199,11 +204,12 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
             prof = next;
             if (REACH && (s & 31) == 0) {
                 // The earliest start found in the wave so far; every cell still to come lies on anti-diagonal
-                // s + 2 or later, so a start before that is final.
+                // s + 2 or later, so a start before that is final. Columns past the query's end are swept too (their
+                // letters stand in) and may reach the target first: they are no start.
                 int diag = 0x7fffffff;
 #pragma unroll
                 for (int i = 0; i < C; ++i)
-                    if (col_r[i] != 0) diag = min(diag, col_r[i] + c0 + i + 1);
+                    if (col_r[i] != 0 && c0 + i + 1 <= cols) diag = min(diag, col_r[i] + c0 + i + 1);
                 if (first_r != 0) diag = min(diag, first_r + first_c);
 #pragma unroll
                 for (int d = 1; d < 64; d <<= 1) diag = min(diag, __shfl_xor(diag, d));
@@ -216,6 +222,10 @@ __global__ __launch_bounds__(64) void score_wave(ScoreArgs a) {
         for (int i = 0; i < C; ++i) {
             const int col = c0 + i + 1;
             if (col > cols) continue;
+            if (LOCAL && KEYED) {
+                col_r[i] = (int)((uint32_t)col_h[i] & ((1u << a.end_bits) - 1u));
+                col_h[i] = (int)((uint32_t)col_h[i] >> a.end_bits);
+            }
             if (LOCAL && col_h[i] > 0 && better_end(col_h[i], col_r[i], col, best_h, best_r, best_c)) {
                 best_h = col_h[i];
                 best_r = col_r[i];
@@ -324,18 +334,18 @@ size_t score_lds(int letters, int rows, bool ring) {
     return (size_t)letters * 64 * 8 + (ring ? (size_t)SCORE_RING_CHUNKS * SCORE_RING_CHUNK : (size_t)rows);
 }
 
-template <int MODE>
+template <int MODE, bool KEYED = false>
 static void launch_mode(const ScoreArgs &a, size_t lds, hipStream_t stream) {
     const dim3 grid(a.count), block(64);
     switch (a.c) {
-        case 1: hipLaunchKernelGGL((score_wave<1, MODE>), grid, block, lds, stream, a); break;
-        case 2: hipLaunchKernelGGL((score_wave<2, MODE>), grid, block, lds, stream, a); break;
-        case 3: hipLaunchKernelGGL((score_wave<3, MODE>), grid, block, lds, stream, a); break;
-        case 4: hipLaunchKernelGGL((score_wave<4, MODE>), grid, block, lds, stream, a); break;
-        case 5: hipLaunchKernelGGL((score_wave<5, MODE>), grid, block, lds, stream, a); break;
-        case 6: hipLaunchKernelGGL((score_wave<6, MODE>), grid, block, lds, stream, a); break;
-        case 7: hipLaunchKernelGGL((score_wave<7, MODE>), grid, block, lds, stream, a); break;
-        default: hipLaunchKernelGGL((score_wave<8, MODE>), grid, block, lds, stream, a); break;
+        case 1: hipLaunchKernelGGL((score_wave<1, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 2: hipLaunchKernelGGL((score_wave<2, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 3: hipLaunchKernelGGL((score_wave<3, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 4: hipLaunchKernelGGL((score_wave<4, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 5: hipLaunchKernelGGL((score_wave<5, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 6: hipLaunchKernelGGL((score_wave<6, MODE, KEYED>), grid, block, lds, stream, a); break;
+        case 7: hipLaunchKernelGGL((score_wave<7, MODE, KEYED>), grid, block, lds, stream, a); break;
+        default: hipLaunchKernelGGL((score_wave<8, MODE, KEYED>), grid, block, lds, stream, a); break;
     }
 }
 
@@ -343,7 +353,10 @@ void launch_score(const ScoreArgs &a, size_t lds, hipStream_t stream) {
     if (a.count <= 0) return;
     switch (a.mode) {
         case SCORE_GLOBAL: launch_mode<SCORE_GLOBAL>(a, lds, stream); break;
-        case SCORE_LOCAL: launch_mode<SCORE_LOCAL>(a, lds, stream); break;
+        case SCORE_LOCAL:
+            if (a.end_bits > 0) launch_mode<SCORE_LOCAL, true>(a, lds, stream);
+            else launch_mode<SCORE_LOCAL>(a, lds, stream);
+            break;
         case SCORE_REACH: launch_mode<SCORE_REACH>(a, lds, stream); break;
         default: launch_mode<SCORE_TRACE>(a, lds, stream); break;
     }

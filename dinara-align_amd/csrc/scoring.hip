@@ -200,6 +200,14 @@ int64_t lds_room(const Batch *b) {
     return b->ctx->lds_limit > profile ? (int64_t)(b->ctx->lds_limit - profile) : 0;
 }
 
+// The table's best score (at least 0).
+int top_score(const Batch *b) {
+    int top = 0;
+    for (int x = 0; x < b->scoring.letters; ++x)
+        for (int y = 0; y < b->scoring.letters; ++y) top = std::max<int>(top, b->scoring.table[x * 32 + y]);
+    return top;
+}
+
 int64_t run_pass(Batch *b, int mode, int reversed, const std::vector<int32_t> &slots, hipEvent_t start,
                  hipEvent_t stop, double *pass_ms, double *walk_ms) {
     DeviceContext *ctx = b->ctx;
@@ -242,6 +250,14 @@ int64_t run_pass(Batch *b, int mode, int reversed, const std::vector<int32_t> &s
             chunk = std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)(ARENA_BUDGET / 4 / a.arena_stride)));
         }
         a.carry_stride = strips > 1 ? 2 * (rows_max[g] + 1) : 0;
+        // A local end's key: the rows' bits below a score no greater than the table's best times the shorter side.
+        a.end_bits = 0;
+        if (mode == SCORE_LOCAL) {
+            int bits = 1;
+            while (bits < 31 && (int64_t(1) << bits) <= rows_max[g]) ++bits;
+            const int64_t best = (int64_t)top_score(b) * std::min(rows_max[g], cols_max[g]);
+            if (best < (int64_t(1) << (32 - bits))) a.end_bits = bits;
+        }
         for (int64_t first = 0; first < count; first += chunk) {
             a.list = list + at[g] + first;
             a.count = (int)std::min(chunk, count - first);

@@ -191,6 +191,27 @@ def test_cigar_runs_across_chunk_edges():
     check_global(refs, qrys, DNA, eqx=False)
 
 
+def test_reach_back_ignores_columns_past_the_query():
+    # A lane sweeps whole strips of columns, those past the query's end on stand-in letters. Under a large match
+    # score those columns reach a local end's score before the real start does; the reach back must not stop on
+    # them (its cutoff counts real columns only). Queries of 3 to 70 letters inside references of 1,000 to 1,500,
+    # with deletions in the embedded copy.
+    rng = random.Random(99)
+    refs, qrys = [], []
+    for _ in range(24):
+        m = rng.randint(3, 70)
+        q = "".join(rng.choice("ACGT") for _ in range(m))
+        copy = "".join(ch + ("".join(rng.choice("ACGT") for _ in range(rng.randint(1, 12))) if rng.random() < 0.2
+                             else "") for ch in q)
+        n = rng.randint(1000, 1500)
+        at = rng.randint(0, n - len(copy))
+        r = "".join(rng.choice("ACGT") for _ in range(at)) + copy
+        refs.append(r + "".join(rng.choice("ACGT") for _ in range(n - len(r))))
+        qrys.append(q)
+    for match in (100, 120, 5):
+        check_local(refs, qrys, da.Scoring.uniform(match, -4, -19, -1))
+
+
 def test_tall_references_go_through_the_ring():
     # A reference longer than LDS stages whole (64 KiB less the profile) is read through a ring of letters
     # (reference tests/test_alignment.mojo:3801-3810: 70 000 letters against a 16-letter query, global and local).
@@ -213,3 +234,6 @@ def test_tall_references_go_through_the_ring():
     check_global(refs, qrys, scoring, eqx=False)
     check_local(refs, qrys, scoring)
     check_local(refs, qrys, DNA)
+    # Local ends are kept as one key a column, score above a row of 17 bits here, while the scores fit the 15 bits
+    # left; a match of 120 against a 600-letter query does not, and the ends are kept by compare and select.
+    check_local(refs, qrys, da.Scoring.uniform(120, -4, -19, -1))
