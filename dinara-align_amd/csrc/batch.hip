@@ -1475,6 +1475,14 @@ dinara_batch *batch_new(int64_t gpu, int64_t kind, int64_t pairs, const char *co
     b->ctx = ctx;
     b->kind = kind;
     b->pairs = std::max<int64_t>(pairs, 0);
+    // Packing threads: as many as asked, but no more than one a half-million letters (a thread's start costs more
+    // than packing fewer letters).
+    {
+        int64_t letters = 0;
+        for (int64_t i = 0; i < b->pairs; ++i)
+            letters += std::max<int64_t>(ref_lens[i], 0) + std::max<int64_t>(qry_lens[i], 0);
+        threads = std::max<int64_t>(1, std::min<int64_t>(threads, letters >> 19));
+    }
     b->threads = threads;
     b->parsed = parsed;
     b->options = parse_options(options);
@@ -2105,6 +2113,14 @@ double dinara_batch_stat(dinara_batch *b, int64_t what) {
     }
 }
 
+// A batch run once (the drop-in calls): the unit path's launches go one by one without timing events, since a
+// captured graph would be instantiated for a single replay.
+void one_shot(dinara_batch *b) {
+    if (!b->unit) return;
+    b->use_graph = false;
+    b->timed = false;
+}
+
 int64_t dinara_gpu_distances(int64_t gpu, int64_t pairs, const char *const *references,
                              const int64_t *reference_lengths, const char *const *queries,
                              const int64_t *query_lengths, const dinara_costs *costs, const dinara_mode *mode,
@@ -2117,6 +2133,7 @@ int64_t dinara_gpu_distances(int64_t gpu, int64_t pairs, const char *const *refe
     dinara_batch *b = batch_new(gpu, DINARA_BATCH_DISTANCES, pairs, references, reference_lengths, queries,
                                 query_lengths, costs, mode, options, threads, &status);
     if (!b) return status;
+    one_shot(b);
     int64_t code = run(b);
     if (code == 0) code = dinara_batch_distances(b, results);
     dinara_batch_free(b);
@@ -2136,6 +2153,7 @@ int64_t dinara_gpu_alignments(int64_t gpu, int64_t pairs, const char *const *ref
     dinara_batch *b = batch_new(gpu, DINARA_BATCH_ALIGNMENTS, pairs, references, reference_lengths, queries,
                                 query_lengths, costs, mode, options, threads, &status);
     if (!b) return status;
+    one_shot(b);
     int64_t code = run(b);
     if (code == 0) code = dinara_batch_alignments(b, alignments, statuses);
     dinara_batch_free(b);
